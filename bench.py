@@ -1,0 +1,161 @@
+"""Benchmark of the MPDATA advection step on one MI355X.
+
+Workload (the single-domain configuration the metric is quoted on): a
+1024 x 1024 x 30 domain, 10 species (qv, theta, qc, qr, qs, qi, qg, ni, nr
+and a passive tracer), mpdata_order 3 with FCT, advect_density on, fractal
+terrain and a 15 m/s westerly with terrain-forced perturbation, w from
+continuity, dt at Courant 0.5.  A step is one advect() call (Courant numbers
+plus every species through all MPDATA passes) with all inputs resident in
+HBM.  value = cell-species updates per second = nx*ny*nz*species*steps / t.
+
+Prints one JSON line.  Options: --steps K --warmup W --gpus N (1 only)
+--variant exact|fast|best --no-cpu (skip the CPU baseline leg).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def measure(case, variant: int, steps: int, warmup: int):
+    from hicar_amd.advection import MpdataAdvection
+    case.options.kernel_variant = variant
+    st = MpdataAdvection(case.domain, case.options)
+    try:
+        st.run(case.dt, warmup)
+        st.sync()
+        st.profile(True)
+        t0 = time.perf_counter()
+        st.run(case.dt, steps)
+        st.sync()
+        t1 = time.perf_counter()
+        stats = st.kernel_stats()
+        st.profile(False)
+    finally:
+        st.close()
+    return t1 - t0, stats
+
+
+def cpu_baseline():
+    """The FP32 oracle (a scalar C restatement of the reference, 1 thread) on
+    a bounded sample of the workload: a 512 x 512 x 30 crop, 10 species, 1
+    step."""
+    import numpy as np
+
+    from hicar_amd import synthetic
+    from oracle.oracle import Oracle
+    case = synthetic.config2(nx=512, ny=512, nz=30, steps=1)
+    orc = Oracle(np.float32)
+    t0 = time.perf_counter()
+    orc.advect(case.domain, case.options, case.dt)
+    t = time.perf_counter() - t0
+    d = case.domain
+    n = d.nx * d.ny * d.nz * len(d.species)
+    return {"value": n / t, "unit": "cell-species updates/s", "cores": 1, "kind": "port",
+            "sample": "512x512x30 crop of the workload, 10 species, order 3 + FCT, 1 step "
+                      f"({t:.1f} s)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--variant", choices=("exact", "fast", "best"), default="best")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--nx", type=int, default=1024)
+    ap.add_argument("--ny", type=int, default=1024)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is a design, not part of this build)",
+              file=sys.stderr)
+        return 2
+
+    from hicar_amd import synthetic
+    log("generating the 1024x1024x30, 10-species workload ...")
+    case = synthetic.config2(nx=args.nx, ny=args.ny)
+    d = case.domain
+    S = len(d.species)
+    cells = d.nx * d.ny * d.nz
+    variants = {"exact": 0, "fast": 1}
+    todo = ["exact", "fast"] if args.variant == "best" else [args.variant]
+    results = {}
+    for name in todo:
+        log(f"variant {name}: {args.warmup} warmup + {args.steps} timed steps ...")
+        try:
+            t, stats = measure(case, variants[name], args.steps, args.warmup)
+        except Exception as e:  # a variant not available in this build
+            log(f"variant {name} unavailable: {e}")
+            continue
+        results[name] = {"seconds": t, "value": cells * S * args.steps / t, "stats": stats}
+        log(f"  {name}: {results[name]['value'] / 1e9:.2f} G cell-species/s, "
+            f"{1e3 * t / args.steps:.2f} ms/step")
+    if not results:
+        log("no variant ran")
+        return 1
+    best = max(results, key=lambda k: results[k]["value"])
+    r = results[best]
+    stats = r["stats"]
+    dom_k = max(stats, key=lambda k: stats[k]["total_ms"])
+    ks = stats[dom_k]
+    avg_s = ks["total_ms"] / ks["launches"] / 1e3
+    bytes_per_launch = ks["alg_bytes"] / ks["launches"]
+    achieved = bytes_per_launch / avg_s
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                "kernel": dom_k, "avg_launch_ms": avg_s * 1e3,
+                "alg_bytes_per_launch": bytes_per_launch}
+    out = {
+        "metric": "MPDATA cell-species updates/s & % HBM roofline, plain FP32 vs batched/mixed",
+        "value": r["value"],
+        "unit": "cell-species updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r["seconds"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"single domain {d.nx}x{d.ny}x{d.nz}, {S} species, "
+                               "mpdata_order 3 + FCT, advect_density, Courant 0.5",
+                   "nx": d.nx, "ny": d.ny, "nz": d.nz, "species": S, "mpdata_order": 3,
+                   "fct": True, "variant": best, "dt": case.dt},
+        "roofline": roofline,
+        "variants": {k: {"value": v["value"], "ms_per_step": 1e3 * v["seconds"] / args.steps,
+                         "kernels": {kk: {"avg_ms": s["total_ms"] / s["launches"],
+                                          "launches": s["launches"],
+                                          "achieved_GBps": s["alg_bytes"] / s["total_ms"] / 1e6}
+                                     for kk, s in v["stats"].items()}}
+                     for k, v in results.items()},
+        "compulsory_hbm_fraction": None,
+    }
+    # compulsory bytes of the whole step (inputs read once, species read and
+    # written once): 4*(5*N + 2*N_u + 2*N_v) + S*2*4*N
+    Nu = (d.nx + 1) * d.nz * d.ny
+    Nv = d.nx * d.nz * (d.ny + 1)
+    b_step = 4 * (5 * cells + 2 * Nu + 2 * Nv) + S * 2 * 4 * cells
+    out["compulsory_hbm_fraction"] = b_step * args.steps / r["seconds"] / HBM_PEAK
+    if not args.no_cpu:
+        log("CPU baseline (oracle, 1 thread) ...")
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,104 @@
+"""ctypes binding of ``libicar_adv.so`` (declared in ``include/icar_adv.h``).
+
+The library must have been built (``python -m hicar_amd.build`` or
+``__graft_entry__.build()``); there is no fallback: loading fails loudly when
+it is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libicar_adv.so")
+
+# enums of icar_adv.h
+FIELD_U, FIELD_V, FIELD_W, FIELD_DENSITY, FIELD_JACOBIAN = 0, 1, 2, 3, 4
+FIELD_JACOBIAN_U, FIELD_JACOBIAN_V, FIELD_JACOBIAN_W, FIELD_ADVECTION_DZ = 5, 6, 7, 8
+NFIELDS = 9
+FIELD_NAMES = ("u", "v", "w", "density", "jacobian", "jacobian_u", "jacobian_v",
+               "jacobian_w", "advection_dz")
+STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
+MAX_SPECIES = 48
+MASK_ALL_FIELDS = 0x1FF
+MASK_ALL_SPECIES = 0xFFFFFFFFFFFF0000
+
+
+def mask_species(s: int) -> int:
+    return 1 << (16 + s)
+
+
+EXPORTS = (
+    "icar_adv_create", "icar_adv_destroy", "icar_adv_bind_field", "icar_adv_bind_species",
+    "icar_adv_upload", "icar_adv_download", "icar_adv_advect", "icar_adv_mass_budget",
+    "icar_adv_max_courant", "icar_adv_last_error", "icar_adv_sync", "icar_adv_apply_bc",
+    "icar_adv_run", "icar_adv_profile", "icar_adv_kernel_count", "icar_adv_kernel_name",
+    "icar_adv_kernel_stats",
+)
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in
+                ("ims", "ime", "kms", "kme", "jms", "jme", "its", "ite", "jts", "jte")] + \
+               [("dx", ctypes.c_float)]
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in
+                ("mpdata_order", "mpdata_fct", "advect_density", "debug", "rk3",
+                 "kernel_variant")]
+
+
+class IcarAdvError(RuntimeError):
+    """A non-zero return code of the C-ABI, with the library's message."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"icar_adv error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build the HIP library first (python -m hicar_amd.build)")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, ip, fp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_float)
+    i3 = ctypes.POINTER(ctypes.c_int)
+    dp = ctypes.POINTER(ctypes.c_double)
+    sig = {
+        "icar_adv_create": ([ctypes.POINTER(Grid), ctypes.POINTER(Opts), ctypes.POINTER(vp)], ip),
+        "icar_adv_destroy": ([vp], ip),
+        "icar_adv_bind_field": ([vp, ip, fp, i3, i3], ip),
+        "icar_adv_bind_species": ([vp, ip, fp, ip, ip], ip),
+        "icar_adv_upload": ([vp, ctypes.c_ulonglong], ip),
+        "icar_adv_download": ([vp, ctypes.c_ulonglong], ip),
+        "icar_adv_advect": ([vp, ctypes.c_float], ip),
+        "icar_adv_mass_budget": ([vp, dp], ip),
+        "icar_adv_max_courant": ([vp, dp], ip),
+        "icar_adv_last_error": ([vp], ctypes.c_char_p),
+        "icar_adv_sync": ([vp], ip),
+        "icar_adv_apply_bc": ([vp, ip, ip], ip),
+        "icar_adv_run": ([vp, ctypes.c_float, ip, ip, ip], ip),
+        "icar_adv_profile": ([vp, ip], ip),
+        "icar_adv_kernel_count": ([], ip),
+        "icar_adv_kernel_name": ([ip], ctypes.c_char_p),
+        "icar_adv_kernel_stats": ([vp, ip, dp, ctypes.POINTER(ctypes.c_longlong), dp], ip),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def check(rc: int, handle=None) -> None:
+    if rc != 0:
+        msg = load().icar_adv_last_error(handle)
+        raise IcarAdvError(rc, msg.decode() if msg else "")

@@ -1,0 +1,156 @@
+"""Host-side mirror of ICAR's advection module for the MPDATA scheme.
+
+``adv_var_request``, ``adv_init`` and ``advect`` keep the names, arguments and
+behaviour of ``src/physics/advection_driver.f90:23-126``; the work happens on
+the GPU through the C-ABI of ``libicar_adv.so``.  Species, winds and metric
+stay resident on the device between calls; ``advect`` updates the host
+arrays in place (as the reference does) unless ``sync_host=False``.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .domain import Domain, Options, kADV_MPDATA, kADV_STD
+
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_fp)
+
+
+class MpdataAdvection:
+    """One device-resident MPDATA state for one domain (one C-ABI handle)."""
+
+    def __init__(self, domain: Domain, options: Options):
+        domain.check()
+        self.lib = _lib.load()
+        self.domain = domain
+        self.options = options
+        g = _lib.Grid(domain.ims, domain.ime, domain.kms, domain.kme, domain.jms, domain.jme,
+                      domain.its, domain.ite, domain.jts, domain.jte, domain.dx)
+        o = _lib.Opts(int(options.adv_options.mpdata_order), int(options.adv_options.MPDATA_FCT),
+                      int(options.parameters.advect_density), int(options.parameters.debug),
+                      int(options.time_options.RK3), int(options.kernel_variant))
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.icar_adv_create(ctypes.byref(g), ctypes.byref(o), ctypes.byref(h)))
+        self.h = h
+        self._keep = []
+        for fid, name in enumerate(_lib.FIELD_NAMES):
+            arr = getattr(domain, name)
+            lb, ub = domain.field_bounds(name)
+            lba = (ctypes.c_int * 3)(*lb)
+            uba = (ctypes.c_int * 3)(*ub)
+            self._check(self.lib.icar_adv_bind_field(self.h, fid, _ptr(arr), lba, uba))
+            self._keep.append(arr)
+        self.names = list(domain.species.keys())
+        if len(self.names) > _lib.MAX_SPECIES:
+            raise ValueError(f"at most {_lib.MAX_SPECIES} advected species")
+        for idx, name in enumerate(self.names):
+            arr = domain.species[name]
+            storage, e = domain.storage.get(name, (_lib.STORE_F32, 0))
+            self._check(self.lib.icar_adv_bind_species(self.h, idx, _ptr(arr), storage, e))
+            self._keep.append(arr)
+        self.upload()
+
+    # -- plumbing ---------------------------------------------------------
+    def _check(self, rc: int) -> None:
+        _lib.check(rc, self.h)
+
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.icar_adv_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- residency --------------------------------------------------------
+    def upload(self, fields: bool = True, species: bool = True) -> None:
+        mask = (_lib.MASK_ALL_FIELDS if fields else 0) | (_lib.MASK_ALL_SPECIES if species else 0)
+        self._check(self.lib.icar_adv_upload(self.h, mask))
+
+    def download(self, species: bool = True, fields: bool = False) -> None:
+        mask = (_lib.MASK_ALL_FIELDS if fields else 0) | (_lib.MASK_ALL_SPECIES if species else 0)
+        self._check(self.lib.icar_adv_download(self.h, mask))
+
+    # -- the step ----------------------------------------------------------
+    def advect(self, dt: float) -> None:
+        self._check(self.lib.icar_adv_advect(self.h, ctypes.c_float(dt)))
+
+    def apply_bc(self, bc_x: int, bc_y: int) -> None:
+        self._check(self.lib.icar_adv_apply_bc(self.h, bc_x, bc_y))
+
+    def run(self, dt: float, nsteps: int, bc_x: int = 0, bc_y: int = 0) -> None:
+        self._check(self.lib.icar_adv_run(self.h, ctypes.c_float(dt), nsteps, bc_x, bc_y))
+
+    def sync(self) -> None:
+        self._check(self.lib.icar_adv_sync(self.h))
+
+    # -- diagnostics ---------------------------------------------------------
+    def mass_budget(self) -> np.ndarray:
+        out = np.zeros(len(self.names), np.float64)
+        self._check(self.lib.icar_adv_mass_budget(
+            self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def max_courant(self) -> np.ndarray:
+        out = np.zeros(3, np.float64)
+        self._check(self.lib.icar_adv_max_courant(
+            self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def profile(self, enable: bool) -> None:
+        self._check(self.lib.icar_adv_profile(self.h, int(enable)))
+
+    def kernel_stats(self) -> dict:
+        out = {}
+        for kid in range(self.lib.icar_adv_kernel_count()):
+            ms = ctypes.c_double()
+            n = ctypes.c_longlong()
+            b = ctypes.c_double()
+            self._check(self.lib.icar_adv_kernel_stats(self.h, kid, ctypes.byref(ms),
+                                                       ctypes.byref(n), ctypes.byref(b)))
+            if n.value:
+                out[self.lib.icar_adv_kernel_name(kid).decode()] = {
+                    "total_ms": ms.value, "launches": n.value, "alg_bytes": b.value}
+        return out
+
+
+def adv_var_request(options: Options) -> None:
+    """advection_driver.f90:46-57 -> adv_std_var_request (advect.f90:94-108):
+    both schemes advect water vapour and potential temperature."""
+    options.advect_vars(["qv", "theta"])
+
+
+def adv_init(domain: Domain, options: Options) -> MpdataAdvection | None:
+    """advection_driver.f90:26-44.  For MPDATA, creates the device state
+    (mpdata_init) and uploads the domain; returns it and keeps it on the
+    domain for ``advect``."""
+    if options.physics.advection == kADV_MPDATA:
+        domain._mpdata = MpdataAdvection(domain, options)
+        return domain._mpdata
+    if options.physics.advection == kADV_STD:
+        raise NotImplementedError("adv=1 (standard advection) is not part of this library")
+    raise ValueError(f"unknown advection scheme {options.physics.advection}")
+
+
+def advect(domain: Domain, options: Options, dt: float, sync_host: bool = True) -> None:
+    """advection_driver.f90:59-126 for adv=2: Courant numbers once, then every
+    advected species in registry order, on the device.  Under RK3 MPDATA does
+    nothing (:112-115).  With ``sync_host`` the host species arrays are
+    updated in place on return, as the reference's are."""
+    if options.physics.advection != kADV_MPDATA:
+        raise NotImplementedError("only adv=2 (MPDATA) is provided by this library")
+    st = getattr(domain, "_mpdata", None)
+    if st is None:
+        raise RuntimeError("adv_init(domain, options) has not been called")
+    st.advect(float(np.float32(dt)))
+    if sync_host:
+        st.download()

@@ -1,0 +1,79 @@
+// Shared definitions of the MPDATA advection library (device and host side).
+//
+// Index convention (identical to ICAR's Fortran column-major arrays
+// (ims:ime, kms:kme, jms:jme), 0-based):  IX(i,k,j) = i + nx*(k + nz*j).
+// Face fields have the mass-grid shape:
+//   U[IX(i,k,j)]  x face between cells i-1 and i (slot i = 0 unused, kept 0)
+//   V[IX(i,k,j)]  y face between rows j-1 and j  (slot j = 0 unused, kept 0)
+//   W[IX(i,k,j)]  z face on top of level k       (k = nz-1 is the model top)
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace icaradv {
+
+struct Dims {
+    int nx, ny, nz;
+};
+
+__host__ __device__ inline size_t ix(const Dims &d, int i, int k, int j)
+{
+    return (size_t)i + (size_t)d.nx * ((size_t)k + (size_t)d.nz * (size_t)j);
+}
+
+// Kernel ids for the profiling accumulators (names in kernel_names()).
+enum KernelId {
+    K_COURANT = 0,
+    K_UPWIND,
+    K_PSEUDO,
+    K_FCT,
+    K_BC,
+    K_REDUCE,
+    K_COPY,
+    K_FUSED_FIRST,
+    K_FUSED_CORR,
+    K_CONVERT,
+    K_COUNT
+};
+
+const char *kernel_name(int id);
+
+// Per-step derived fields written by the Courant kernel (species invariant).
+struct StepFields {
+    float *U, *V, *W;  // Courant numbers (mpdata_compute_wind)
+    float *G;          // jacobian*density            (adv_mpdata.f90:87,374)
+    float *Gz;         // advection_dz*jacobian*density (adv_mpdata.f90:91-97)
+    float *Wd;         // W/advection_dz              (adv_mpdata.f90:374)
+};
+
+// Raw domain inputs on the device (ICAR shapes; u/jacobian_u have nx+1 per
+// row, v/jacobian_v ny+1 rows).
+struct DomainFields {
+    const float *u, *v, *w, *density, *jaco, *jaco_u, *jaco_v, *jaco_w, *dz;
+};
+
+// ---- exact kernels (reference IEEE operation order), kernels_exact.hip ----
+void launch_courant(hipStream_t s, Dims d, float dx, float dt, int advect_density,
+                    const DomainFields &f, const StepFields &o);
+void launch_upwind(hipStream_t s, Dims d, const float *qin, const float *U, const float *V,
+                   const float *W, const float *G, const float *Gz, float *q);
+void launch_pseudo(hipStream_t s, Dims d, const float *q1, const StepFields &sf, const float *dz,
+                   float *u2, float *v2, float *w2);
+void launch_fct(hipStream_t s, Dims d, const float *q0, const float *q1, const float *u2,
+                const float *v2, const float *w2, float *u2l, float *v2l, float *w2l);
+void launch_bc(hipStream_t s, Dims d, float *q, int bc_x, int bc_y);
+// FP64 mass budget partial sums: writes nblocks doubles to partial.
+int mass_partials_blocks(Dims d);
+void launch_mass(hipStream_t s, Dims d, const float *q, const float *jaco, const float *density,
+                 const float *dz, double *partial);
+// FP64 Courant maxima partials: 3 doubles per block.
+void launch_courant_max(hipStream_t s, Dims d, const float *U, const float *V, const float *Wd,
+                        double *partial);
+void launch_copy(hipStream_t s, size_t n, const float *src, float *dst);
+// Divergence check (debug): counts cells with |div| > 1e-3; out[0]=count, out[1]=max.
+void launch_divergence(hipStream_t s, Dims d, const float *U, const float *V, const float *W,
+                       const float *dz, double *partial);
+
+}  // namespace icaradv

@@ -1,0 +1,174 @@
+/*
+ * icar_adv.h -- C-ABI of the MI355X MPDATA advection library (libicar_adv.so).
+ *
+ * It replaces, for the MPDATA scheme (&physics adv=2), the three calls that
+ * ICAR's advection module exposes:
+ *   adv_init(domain, options)      src/physics/advection_driver.f90:26-44
+ *                                  -> mpdata_init, src/physics/adv_mpdata.f90:415-442
+ *   advect(domain, options, dt)    src/physics/advection_driver.f90:59-126
+ *                                  (called at src/main/time_step.f90:373)
+ *   adv_var_request(options)       src/physics/advection_driver.f90:46-57
+ *                                  (species registry; stays on the Fortran side)
+ * Plain C types only: pointers, ints, floats, doubles.  Every entry point
+ * returns 0 on success, a negative ICAR_ADV_E* code for argument/shape errors,
+ * or a positive hipError_t value for device errors; no C++ exception crosses
+ * the boundary.  icar_adv_last_error() returns the message of the last failure
+ * on a handle (or of the last failed create when the handle is NULL).
+ *
+ * Layout: every array is Fortran column-major with ICAR's bounds
+ * (ims:ime, kms:kme, jms:jme), i fastest.  u and jacobian_u extend to ime+1,
+ * v and jacobian_v to jme+1 (domain_obj.f90:1916-1931).
+ *
+ * Residency: species, winds, density and metric stay on the device between
+ * calls.  icar_adv_upload / icar_adv_download are the only host<->device
+ * synchronisation points for the data; icar_adv_advect only enqueues work on
+ * the handle's stream and returns.
+ *
+ * Threading: one handle per process/image; calls on one handle must not be
+ * concurrent (advect is called outside any OpenMP region, time_step.f90:373).
+ */
+#ifndef ICAR_ADV_H
+#define ICAR_ADV_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Memory and tile bounds of the local domain, as mpdata_init copies them from
+ * domain_t (adv_mpdata.f90:422-431), and the grid spacing domain%dx
+ * (adv_mpdata.f90:486). */
+typedef struct {
+    int ims, ime, kms, kme, jms, jme;
+    int its, ite, jts, jte;
+    float dx;
+} icar_adv_grid;
+
+/* Namelist switches the MPDATA path reads (options_t):
+ *   mpdata_order    &adv_parameters mpdata_order   (options_obj.f90:1568,1581; default 2)
+ *   mpdata_fct      &adv_parameters MPDATA_FCT     (options_obj.f90:1568,1580; default .true.)
+ *   advect_density  &parameters advect_density     (options_obj.f90:1025; default .false.)
+ *   debug           &parameters debug: runs the divergence check of
+ *                   mpdata_compute_wind (adv_mpdata.f90:500-502) and syncs
+ *   rk3             &time_parameters RK3: MPDATA under RK3 does nothing
+ *                   (advection_driver.f90:112-115); advect returns 0
+ *   kernel_variant  no reference counterpart: 0 = "exact" kernels that keep
+ *                   the reference's IEEE operation order (bit-identical to a
+ *                   strict FP32 build), 1 = "fast" fused/batched kernels. */
+typedef struct {
+    int mpdata_order;
+    int mpdata_fct;
+    int advect_density;
+    int debug;
+    int rk3;
+    int kernel_variant;
+} icar_adv_opts;
+
+/* Fields bound with icar_adv_bind_field (names of domain_t components). */
+enum {
+    ICAR_ADV_FIELD_U = 0,        /* domain%u%data_3d        (ims:ime+1, kms:kme, jms:jme) */
+    ICAR_ADV_FIELD_V = 1,        /* domain%v%data_3d        (ims:ime, kms:kme, jms:jme+1) */
+    ICAR_ADV_FIELD_W = 2,        /* domain%w%data_3d        mass grid */
+    ICAR_ADV_FIELD_DENSITY = 3,  /* domain%density%data_3d  mass grid */
+    ICAR_ADV_FIELD_JACOBIAN = 4, /* domain%jacobian         mass grid */
+    ICAR_ADV_FIELD_JACOBIAN_U = 5,
+    ICAR_ADV_FIELD_JACOBIAN_V = 6,
+    ICAR_ADV_FIELD_JACOBIAN_W = 7,
+    ICAR_ADV_FIELD_ADVECTION_DZ = 8,
+    ICAR_ADV_NFIELDS = 9
+};
+
+/* Device storage of a species (the host side is always 4-byte real). */
+enum { ICAR_ADV_STORE_F32 = 0, ICAR_ADV_STORE_BF16 = 1, ICAR_ADV_STORE_F16 = 2 };
+
+/* Error codes (negative); positive values are hipError_t. */
+enum {
+    ICAR_ADV_OK = 0,
+    ICAR_ADV_EINVAL = -1,   /* bad argument or NULL pointer */
+    ICAR_ADV_ESHAPE = -2,   /* array bounds do not match the grid */
+    ICAR_ADV_EUNBOUND = -3, /* a required field or species was not bound */
+    ICAR_ADV_ERANGE = -4,   /* index or size beyond a documented limit */
+    ICAR_ADV_ENOMEM = -5,   /* device allocation failed */
+    ICAR_ADV_ECOURANT = -6  /* debug check: divergence/Courant out of range */
+};
+
+#define ICAR_ADV_MAX_SPECIES 48
+/* upload/download masks: bits 0..8 select fields, bit 16+s selects species s */
+#define ICAR_ADV_MASK_FIELD(f) (1ULL << (f))
+#define ICAR_ADV_MASK_ALL_FIELDS 0x1FFULL
+#define ICAR_ADV_MASK_SPECIES(s) (1ULL << (16 + (s)))
+#define ICAR_ADV_MASK_ALL_SPECIES 0xFFFFFFFFFFFF0000ULL
+
+/* = adv_init/mpdata_init (advection_driver.f90:26-44, adv_mpdata.f90:415-442).
+ * Validates bounds (nx, ny, nz >= 3) and allocates the device mirrors and
+ * work space on the current HIP device.  Unlike mpdata_init it keeps its
+ * state in the handle, so several domains can coexist in one process. */
+int icar_adv_create(const icar_adv_grid *grid, const icar_adv_opts *opts, void **handle);
+
+/* Frees every device allocation of the handle. */
+int icar_adv_destroy(void *handle);
+
+/* Registers the host array of a domain_t field (c_loc plus Fortran bounds
+ * lb/ub in (i,k,j) order).  The array must outlive the handle. */
+int icar_adv_bind_field(void *handle, int field_id, float *host, const int lb[3], const int ub[3]);
+
+/* Registers advected species idx (0-based position in domain%adv_vars,
+ * domain_obj.f90:68-89) with its host array (mass-grid bounds of the
+ * domain).  storage selects the device format; pow2_exp is the per-species
+ * power-of-two scale e: the device holds q * 2^-e (ignored for FP32). */
+int icar_adv_bind_species(void *handle, int idx, float *host, int storage, int pow2_exp);
+
+/* Copies the selected host arrays to the device (after CPU physics touched
+ * them) / device arrays to the host (before microphysics, output, restart).
+ * Both synchronise the handle's stream. */
+int icar_adv_upload(void *handle, unsigned long long mask);
+int icar_adv_download(void *handle, unsigned long long mask);
+
+/* = advect(domain, options, dt) for adv=2: Courant numbers once
+ * (mpdata_compute_wind, adv_mpdata.f90:477-504), then every bound species in
+ * index order through the MPDATA passes (mpdata_advect3d, :352-413).
+ * Enqueues on the handle's stream and returns. */
+int icar_adv_advect(void *handle, float dt);
+
+/* Per-species FP64 budget sum(q * jacobian * density * advection_dz) over the
+ * cells advect updates (ims+1:ime-1, all k, jms+1:jme-1); per_species holds
+ * one double per bound species.  Synchronises. */
+int icar_adv_mass_budget(void *handle, double *per_species);
+
+/* FP64 maxima over the interior of |U_m|, |V_m| and |W_m/dz| for the dt of
+ * the last advect call (the cfl_strictness=3 measure of compute_dt,
+ * time_step.f90:98-127, expressed as Courant numbers).  Synchronises. */
+int icar_adv_max_courant(void *handle, double out[3]);
+
+/* Message of the last failure on this handle (handle may be NULL). */
+const char *icar_adv_last_error(void *handle);
+
+/* ---- Extensions with no reference counterpart ---------------------------- */
+
+/* Waits for all work enqueued on the handle's stream. */
+int icar_adv_sync(void *handle);
+
+/* Lateral boundary conditions on the outer ring of every species, applied on
+ * the device between advect calls.  In ICAR the ring is filled by forcing and
+ * halo exchange (time_step.f90:319,362-368); this does what the reference's
+ * MPDATA test does (test_mpdata.f90:96-97).  bc_x: 0 fixed, 1 zero-gradient
+ * outflow at i = ime, 2 periodic; bc_y: 0 fixed, 2 periodic.  x first. */
+int icar_adv_apply_bc(void *handle, int bc_x, int bc_y);
+
+/* Runs nsteps x (advect(dt), then apply_bc(bc_x, bc_y)) on the stream. */
+int icar_adv_run(void *handle, float dt, int nsteps, int bc_x, int bc_y);
+
+/* Kernel timing with HIP events recorded on the handle's stream around each
+ * launch.  enable != 0 starts a fresh accumulation.  Kernel ids and names:
+ * icar_adv_kernel_name(id) for id in [0, icar_adv_kernel_count()). */
+int icar_adv_profile(void *handle, int enable);
+int icar_adv_kernel_count(void);
+const char *icar_adv_kernel_name(int id);
+/* total milliseconds, number of launches and the algorithmic HBM bytes of
+ * all timed launches of kernel id since profiling was enabled.  Synchronises. */
+int icar_adv_kernel_stats(void *handle, int id, double *total_ms, long long *launches,
+                          double *alg_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ICAR_ADV_H */

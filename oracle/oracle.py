@@ -1,0 +1,144 @@
+"""ctypes wrapper of the C oracle (oracle/mpdata_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the
+CPU-baseline leg of bench.py, never by the product package.  Arrays follow
+hicar_amd.domain: C-order (ny, nz, nx) numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+class Oracle:
+    """FP32 (dtype=np.float32) or FP64 (np.float64) restatement."""
+
+    def __init__(self, dtype=np.float32):
+        self.dtype = np.dtype(dtype)
+        name = "liboracle_f32.so" if self.dtype == np.float32 else "liboracle_f64.so"
+        path = os.path.join(HERE, name)
+        if not os.path.exists(path):
+            build()
+        self.lib = ctypes.CDLL(path)
+        self.real = ctypes.c_float if self.dtype == np.float32 else ctypes.c_double
+        assert self.lib.oracle_real_size() == self.dtype.itemsize
+
+    # -- helpers -----------------------------------------------------------
+    def _p(self, a: np.ndarray):
+        assert a.dtype == self.dtype and a.flags.c_contiguous, (a.dtype, a.flags)
+        return a.ctypes.data_as(ctypes.c_void_p)
+
+    def arr(self, a) -> np.ndarray:
+        return np.ascontiguousarray(a, dtype=self.dtype)
+
+    @staticmethod
+    def _dims(q: np.ndarray):
+        ny, nz, nx = q.shape
+        return nx, ny, nz
+
+    # -- restated reference routines -------------------------------------
+    def compute_wind(self, dom, dt, advect_density):
+        nx, ny, nz = dom.nx, dom.ny, dom.nz
+        U = np.zeros(dom.shape, self.dtype)
+        V = np.zeros(dom.shape, self.dtype)
+        W = np.zeros(dom.shape, self.dtype)
+        a = [self.arr(getattr(dom, n)) for n in
+             ("u", "v", "w", "density", "jacobian_u", "jacobian_v", "jacobian_w")]
+        self.lib.oracle_compute_wind(nx, ny, nz, self.real(dom.dx), self.real(dt),
+                                     int(advect_density), *[self._p(x) for x in a],
+                                     self._p(U), self._p(V), self._p(W))
+        return U, V, W
+
+    def upwind(self, qin, U, V, W, rho, dz, jaco):
+        nx, ny, nz = self._dims(qin)
+        q = np.empty_like(qin)
+        self.lib.oracle_upwind(nx, ny, nz, self._p(qin), self._p(U), self._p(V), self._p(W),
+                               self._p(rho), self._p(q), self._p(dz), self._p(jaco))
+        return q
+
+    def fluxes(self, q, U, V, Wd, G):
+        nx, ny, nz = self._dims(q)
+        u2, v2, w2 = (np.empty_like(q) for _ in range(3))
+        self.lib.oracle_fluxes(nx, ny, nz, self._p(q), self._p(U), self._p(V), self._p(Wd),
+                               self._p(u2), self._p(v2), self._p(w2), self._p(G))
+        return u2, v2, w2
+
+    def fct_core(self, q1, l, U2, flux_is_w):
+        q1, l, U2 = self.arr(q1), self.arr(l), self.arr(U2).copy()
+        n = q1.shape[0]
+        assert l.shape == (n,) and U2.shape == (n - 1,) and n >= 3
+        f = np.empty(n, self.dtype)
+        self.lib.oracle_fct_core(n, self._p(q1), self._p(l), self._p(U2), int(flux_is_w),
+                                 self._p(f))
+        return U2
+
+    def flux_limiter(self, q0, q1, u2, v2, w2):
+        nx, ny, nz = self._dims(q0)
+        u2, v2, w2 = u2.copy(), v2.copy(), w2.copy()
+        self.lib.oracle_flux_limiter(nx, ny, nz, self._p(q0), self._p(q1), self._p(u2),
+                                     self._p(v2), self._p(w2))
+        return u2, v2, w2
+
+    def advect3d(self, q, rho, jaco, dz, U, V, W, order, fct):
+        nx, ny, nz = self._dims(q)
+        q = q.copy()
+        rc = self.lib.oracle_advect3d(nx, ny, nz, self._p(q), self._p(rho), self._p(jaco),
+                                      self._p(dz), self._p(U), self._p(V), self._p(W),
+                                      int(order), int(fct))
+        assert rc == 0, rc
+        return q
+
+    def advect(self, dom, options, dt, species=None):
+        """advection_driver.f90:59-126 on copies of the domain species (or
+        the given dict); returns the dict of advanced species."""
+        src = dom.species if species is None else species
+        names = list(src.keys())
+        qs = [self.arr(src[n]).copy() for n in names]
+        ptrs = (ctypes.c_void_p * len(qs))(*[q.ctypes.data for q in qs])
+        f = [self.arr(getattr(dom, n)) for n in
+             ("u", "v", "w", "density", "jacobian", "jacobian_u", "jacobian_v", "jacobian_w",
+              "advection_dz")]
+        rc = self.lib.oracle_advect(
+            dom.nx, dom.ny, dom.nz, self.real(dom.dx), self.real(dt),
+            int(options.adv_options.mpdata_order), int(options.adv_options.MPDATA_FCT),
+            int(options.parameters.advect_density), int(options.time_options.RK3),
+            *[self._p(x) for x in f], len(qs), ptrs)
+        assert rc == 0, rc
+        return dict(zip(names, qs))
+
+    def run(self, dom, options, dt, steps, bc_x=0, bc_y=0, species=None):
+        """steps x (advect, then ring boundary conditions)."""
+        cur = {n: self.arr(a).copy() for n, a in (species or dom.species).items()}
+        for _ in range(steps):
+            cur = self.advect(dom, options, dt, cur)
+            for q in cur.values():
+                self.apply_bc(q, bc_x, bc_y)
+        return cur
+
+    def divergence(self, U, V, W, dz):
+        nx, ny, nz = self._dims(U)
+        mx = ctypes.c_double()
+        self.lib.oracle_divergence.restype = ctypes.c_long
+        n = self.lib.oracle_divergence(nx, ny, nz, self._p(U), self._p(V), self._p(W),
+                                       self._p(dz), ctypes.byref(mx))
+        return int(n), mx.value
+
+    def mass(self, q, jaco, rho, dz) -> float:
+        nx, ny, nz = self._dims(q)
+        self.lib.oracle_mass.restype = ctypes.c_double
+        return self.lib.oracle_mass(nx, ny, nz, self._p(q), self._p(jaco), self._p(rho),
+                                    self._p(dz))
+
+    def apply_bc(self, q, bc_x, bc_y):
+        nx, ny, nz = self._dims(q)
+        self.lib.oracle_apply_bc(nx, ny, nz, self._p(q), int(bc_x), int(bc_y))
+        return q

@@ -1,0 +1,62 @@
+"""CPU checks of the C-ABI boundary (no GPU work): the library loads, exports
+every entry point include/icar_adv.h declares, and rejects bad arguments in
+host code before touching the device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from hicar_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "icar_adv.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(icar_adv_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    syms = declared_symbols()
+    assert len(syms) >= 17
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(_lib.EXPORTS)
+
+
+def test_create_rejects_small_grid_without_device():
+    lib = _lib.load()
+    g = _lib.Grid(1, 2, 1, 5, 1, 5, 1, 2, 1, 5, 1000.0)
+    o = _lib.Opts(2, 1, 0, 0, 0, 0)
+    h = ctypes.c_void_p()
+    rc = lib.icar_adv_create(ctypes.byref(g), ctypes.byref(o), ctypes.byref(h))
+    assert rc == -2 and not h.value
+    assert b"at least 3" in lib.icar_adv_last_error(None)
+
+
+@pytest.mark.parametrize("field,val", [("mpdata_order", -1), ("kernel_variant", 7)])
+def test_create_rejects_bad_options(field, val):
+    lib = _lib.load()
+    g = _lib.Grid(1, 8, 1, 5, 1, 6, 3, 6, 3, 4, 1000.0)
+    o = _lib.Opts(2, 1, 0, 0, 0, 0)
+    setattr(o, field, val)
+    h = ctypes.c_void_p()
+    assert lib.icar_adv_create(ctypes.byref(g), ctypes.byref(o), ctypes.byref(h)) == -1
+
+
+def test_null_arguments():
+    lib = _lib.load()
+    assert lib.icar_adv_create(None, None, None) == -1
+    assert lib.icar_adv_advect(None, ctypes.c_float(1.0)) == -1
+    assert lib.icar_adv_destroy(None) == 0
+
+
+def test_kernel_names():
+    lib = _lib.load()
+    n = lib.icar_adv_kernel_count()
+    names = [lib.icar_adv_kernel_name(i).decode() for i in range(n)]
+    assert "upwind" in names and "fct" in names and len(set(names)) == n
+    assert lib.icar_adv_kernel_name(n) == b""
