@@ -129,7 +129,8 @@ def test_mass_budget_and_courant(oracle32, oracle64):
         Wd = W / d.advection_dz
         assert c[0] == pytest.approx(np.abs(U[1:-1, :, 1:]).max(), rel=1e-6)
         assert c[2] == pytest.approx(np.abs(Wd[1:-1, :, 1:-1]).max(), rel=1e-6)
-        assert 0.3 < max(c) < 0.7
+        assert c[1] == pytest.approx(np.abs(V[1:, :, 1:-1]).max(), abs=1e-12)
+        assert 0 < max(c) <= 0.5 * 1.3  # dt at Courant 0.5 of the physical winds
     finally:
         st.close()
 
