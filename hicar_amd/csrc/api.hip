@@ -51,6 +51,7 @@ struct Handle {
     float *stage = nullptr;  // FP32 staging for reduced-precision species
     SpeciesSlot sp[ICAR_ADV_MAX_SPECIES];
     int nspecies = 0;  // 1 + highest bound index
+    int compact[ICAR_ADV_MAX_SPECIES] = {};  // bound-species position in the fused state
     hipStream_t stream = nullptr;
     float last_dt = 0.0f;
     bool have_dt = false;
@@ -225,6 +226,13 @@ int species_f32_ptr(Handle *h, int s, float **p)
     return 0;
 }
 
+// Device buffer that holds the current state of species s.
+void *species_dev(Handle *h, int s)
+{
+    if (h->fast) return fast_species_ptr(h->fast, h->compact[s]);
+    return h->sp[s].dev;
+}
+
 // One species through mpdata_advect3d with the exact kernels.
 int advect_species_exact(Handle *h, float *q)
 {
@@ -319,10 +327,12 @@ int do_advect(Handle *h, float dt)
 
 int sync_fast_species(Handle *h)
 {
-    // The fused path keeps its own species table; refresh it after binding.
+    // The fused path keeps its own species table (and a second buffer per
+    // species that it alternates with); refresh it after binding.
     std::vector<FastSpecies> v;
     for (int s = 0; s < h->nspecies; s++) {
         if (!h->sp[s].bound) continue;
+        h->compact[s] = (int)v.size();
         v.push_back({h->sp[s].dev, h->sp[s].storage, h->sp[s].pow2});
     }
     int rc = fast_set_species(h->fast, v.data(), (int)v.size());
@@ -484,13 +494,13 @@ int icar_adv_upload(void *handle, unsigned long long mask)
         SpeciesSlot &sl = h->sp[s];
         if (!(mask & ICAR_ADV_MASK_SPECIES(s)) || !sl.bound) continue;
         if (sl.storage == ICAR_ADV_STORE_F32) {
-            HIPCHK(h, hipMemcpyAsync(sl.dev, sl.host, h->N * sizeof(float), hipMemcpyHostToDevice,
+            HIPCHK(h, hipMemcpyAsync(species_dev(h, s), sl.host, h->N * sizeof(float), hipMemcpyHostToDevice,
                                      h->stream));
         } else {
             HIPCHK(h, hipMemcpyAsync(h->stage, sl.host, h->N * sizeof(float),
                                      hipMemcpyHostToDevice, h->stream));
             timed(h, K_CONVERT, 6.0 * h->N,
-                  [&] { launch_pack(h->stream, h->N, h->stage, sl.dev, sl.storage, sl.pow2); });
+                  [&] { launch_pack(h->stream, h->N, h->stage, species_dev(h, s), sl.storage, sl.pow2); });
         }
     }
     HIPCHK(h, hipGetLastError());
@@ -513,11 +523,11 @@ int icar_adv_download(void *handle, unsigned long long mask)
         SpeciesSlot &sl = h->sp[s];
         if (!(mask & ICAR_ADV_MASK_SPECIES(s)) || !sl.bound) continue;
         if (sl.storage == ICAR_ADV_STORE_F32) {
-            HIPCHK(h, hipMemcpyAsync(sl.host, sl.dev, h->N * sizeof(float), hipMemcpyDeviceToHost,
+            HIPCHK(h, hipMemcpyAsync(sl.host, species_dev(h, s), h->N * sizeof(float), hipMemcpyDeviceToHost,
                                      h->stream));
         } else {
             timed(h, K_CONVERT, 6.0 * h->N,
-                  [&] { launch_unpack(h->stream, h->N, sl.dev, h->stage, sl.storage, sl.pow2); });
+                  [&] { launch_unpack(h->stream, h->N, species_dev(h, s), h->stage, sl.storage, sl.pow2); });
             HIPCHK(h, hipMemcpyAsync(sl.host, h->stage, h->N * sizeof(float),
                                      hipMemcpyDeviceToHost, h->stream));
             HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -549,9 +559,9 @@ int icar_adv_mass_budget(void *handle, double *per_species)
     for (int s = 0; s < h->nspecies; s++) {
         SpeciesSlot &sl = h->sp[s];
         if (!sl.bound) continue;
-        const float *q = static_cast<const float *>(sl.dev);
+        const float *q = static_cast<const float *>(species_dev(h, s));
         if (sl.storage != ICAR_ADV_STORE_F32) {
-            launch_unpack(h->stream, h->N, sl.dev, h->stage, sl.storage, sl.pow2);
+            launch_unpack(h->stream, h->N, species_dev(h, s), h->stage, sl.storage, sl.pow2);
             q = h->stage;
         }
         timed(h, K_REDUCE, 4.0 * 4 * cells(h), [&] {
@@ -619,9 +629,9 @@ int icar_adv_apply_bc(void *handle, int bc_x, int bc_y)
         if (!sl.bound) continue;
         timed(h, K_BC, 0.0, [&] {
             if (sl.storage == ICAR_ADV_STORE_F32)
-                launch_bc(h->stream, h->d, static_cast<float *>(sl.dev), bc_x, bc_y);
+                launch_bc(h->stream, h->d, static_cast<float *>(species_dev(h, s)), bc_x, bc_y);
             else
-                launch_bc16(h->stream, h->d, sl.dev, bc_x, bc_y);
+                launch_bc16(h->stream, h->d, species_dev(h, s), bc_x, bc_y);
         });
     }
     HIPCHK(h, hipGetLastError());

@@ -19,6 +19,8 @@ FastState *fast_create(Dims d);
 void fast_destroy(FastState *st);
 int fast_set_species(FastState *st, const FastSpecies *sp, int n);
 const char *fast_error(FastState *st);
+// Device buffer holding the current state of species s (in bind order).
+void *fast_species_ptr(FastState *st, int s);
 
 using TimedFn = std::function<void(int id, double bytes, const std::function<void()> &)>;
 int fast_advect(FastState *st, hipStream_t s, Dims d, const StepFields &sf, const DomainFields &f,

@@ -1,72 +1,736 @@
-// Fused species-batched path (kernel_variant 1): not yet built; the entry
-// point reports a clear error.  Reduced-precision storage helpers live here.
+// Fused, species-batched MPDATA path (kernel_variant 1).
+//
+// One advect() step of mpdata_order n (adv_mpdata.f90:352-413) becomes
+//   k_inv      1/(jaco*rho) and 1/(dz*jaco*rho) once per step
+//   k_first    the donor-cell pass (upwind_advection, :43-103) for every
+//              species in one launch, the Courant numbers and reciprocals
+//              loaded once per cell and reused across species
+//   k_corr     one launch per corrective pass (iord >= 2) for every species:
+//              antidiffusive pseudo-velocities (mpdata_fluxes, :105-252),
+//              the 1-D FCT limiter in x, y and z (flux_limiter, :254-350 and
+//              adv_mpdata_FCT_core.f90) and the donor-cell flux divergence
+//              with the limited velocities, fused in one kernel.
+// k_corr marches upward through the levels of a 60 x 12 column tile (halo 2,
+// 64 x 16 columns staged in LDS); x/y neighbours come from LDS planes, the z
+// direction (including the column-wise FCT) stays in registers of the thread
+// that owns the column.  Species are independent inside advect, so a launch
+// covers (tile, species) pairs; blocks are mapped so that the species of one
+// tile run on the same XCD and share its L2.
+//
+// Arithmetic is FP32 with reciprocals (v_rcp_f32) in place of divisions and
+// contraction allowed: results agree with the reference order to rounding,
+// not bit for bit (the exact kernels keep the bit pattern).
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/icar_adv.h"
 #include "fast.hpp"
 
 namespace icaradv {
 
-struct FastState {
-    Dims d;
-    std::string err;
+namespace {
+
+// ---------------------------------------------------------------------------
+// species storage: FP32 or 16-bit (BF16 / FP16) holding value * 2^-e
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float ld_q(const void *p, size_t i, int kind, float scale)
+{
+    if (kind == ICAR_ADV_STORE_F32) return static_cast<const float *>(p)[i];
+    const uint16_t b = static_cast<const uint16_t *>(p)[i];
+    const float v = (kind == ICAR_ADV_STORE_BF16) ? __uint_as_float((uint32_t)b << 16)
+                                                  : __half2float(__ushort_as_half(b));
+    return v * scale;
+}
+
+__device__ __forceinline__ uint16_t to16(float v, int kind, float inv_scale)
+{
+    const float x = v * inv_scale;
+    if (kind == ICAR_ADV_STORE_BF16) {
+        uint32_t u = __float_as_uint(x);
+        if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+        u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+        return (uint16_t)(u >> 16);
+    }
+    return __half_as_ushort(__float2half_rn(x));
+}
+
+__device__ __forceinline__ void st_q(void *p, size_t i, int kind, float inv_scale, float v)
+{
+    if (kind == ICAR_ADV_STORE_F32)
+        static_cast<float *>(p)[i] = v;
+    else
+        static_cast<uint16_t *>(p)[i] = to16(v, kind, inv_scale);
+}
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// donor-cell flux ((U+|U|) l + (U-|U|) r)/2 = U * (upwind value)
+__device__ __forceinline__ float dflux(float l, float r, float U) { return U * (U > 0.0f ? l : r); }
+
+__device__ __forceinline__ float max3(float a, float b, float c) { return fmaxf(fmaxf(a, b), c); }
+__device__ __forceinline__ float min3(float a, float b, float c) { return fminf(fminf(a, b), c); }
+
+// One pass of one species: where to read q0 (and q1) and where to write.
+struct SpecPass {
+    const void *q0;   // field before the pass (storage kind0)
+    const float *q1;  // field after the previous pass (FP32), or nullptr if equal to q0
+    void *out;        // result (storage kindo)
+    int kind0, kindo;
+    float scale0, inv_scaleo;
 };
+
+// ---------------------------------------------------------------------------
+// reciprocals of the divergence denominators, once per step
+// ---------------------------------------------------------------------------
+__global__ void k_inv(size_t n, const float *__restrict__ G, const float *__restrict__ Gz,
+                      float *__restrict__ invG, float *__restrict__ invGz)
+{
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * blockDim.x) {
+        invG[p] = 1.0f / G[p];
+        invGz[p] = 1.0f / Gz[p];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// donor-cell pass for every species (upwind_advection, adv_mpdata.f90:43-103)
+// ---------------------------------------------------------------------------
+constexpr int FBX = 64, FBY = 4;
+
+__global__ void __launch_bounds__(FBX *FBY)
+    k_first(Dims d, const float *__restrict__ U, const float *__restrict__ V,
+            const float *__restrict__ W, const float *__restrict__ invG,
+            const float *__restrict__ invGz, const SpecPass *__restrict__ sp, int nspec)
+{
+    const int i = blockIdx.x * FBX + threadIdx.x;
+    const int j = blockIdx.z * FBY + threadIdx.y;
+    const int k = blockIdx.y;
+    if (i >= d.nx || j >= d.ny) return;
+    const size_t sk = (size_t)d.nx, sj = (size_t)d.nx * d.nz;
+    const size_t c = (size_t)i + sk * ((size_t)k + (size_t)d.nz * j);
+    const bool ring = i == 0 || i == d.nx - 1 || j == 0 || j == d.ny - 1;
+    float Ul = 0, Ur = 0, Vs = 0, Vn = 0, Wt = 0, Wb = 0, iG = 0, iGz = 0;
+    if (!ring) {
+        Ul = U[c];
+        Ur = U[c + 1];
+        Vs = V[c];
+        Vn = V[c + sj];
+        Wt = W[c];
+        Wb = k > 0 ? W[c - sk] : 0.0f;
+        iG = invG[c];
+        iGz = invGz[c];
+    }
+    for (int s = 0; s < nspec; s++) {
+        const SpecPass p = sp[s];
+        const float qc = ld_q(p.q0, c, p.kind0, p.scale0);
+        if (ring) {  // :61-63
+            st_q(p.out, c, p.kindo, p.inv_scaleo, qc);
+            continue;
+        }
+        const float qw = ld_q(p.q0, c - 1, p.kind0, p.scale0);
+        const float qe = ld_q(p.q0, c + 1, p.kind0, p.scale0);
+        const float qs = ld_q(p.q0, c - sj, p.kind0, p.scale0);
+        const float qn = ld_q(p.q0, c + sj, p.kind0, p.scale0);
+        const float fr = dflux(qc, qe, Ur), fl = dflux(qw, qc, Ul);
+        const float fn = dflux(qc, qn, Vn), fs = dflux(qs, qc, Vs);
+        float q = qc - ((fr - fl) + (fn - fs)) * iG;
+        if (k == 0) {
+            const float qu = ld_q(p.q0, c + sk, p.kind0, p.scale0);
+            q = q - dflux(qc, qu, Wt) * iGz;
+        } else if (k == d.nz - 1) {
+            const float qd = ld_q(p.q0, c - sk, p.kind0, p.scale0);
+            q = q - (qc * Wt - dflux(qd, qc, Wb)) * iGz;  // top flux not upwinded (:96-97)
+        } else {
+            const float qu = ld_q(p.q0, c + sk, p.kind0, p.scale0);
+            const float qd = ld_q(p.q0, c - sk, p.kind0, p.scale0);
+            q = q - (dflux(qc, qu, Wt) - dflux(qd, qc, Wb)) * iGz;
+        }
+        st_q(p.out, c, p.kindo, p.inv_scaleo, q);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fused corrective pass
+// ---------------------------------------------------------------------------
+constexpr int CX = 64;           // staged tile width (one wavefront per row)
+constexpr int CY = 16;           // staged tile rows
+constexpr int CH = 2;            // halo
+constexpr int CTI = CX - 2 * CH; // 60 output columns
+constexpr int CTJ = CY - 2 * CH; // 12 output rows
+constexpr int CNT = 256;         // 4 wavefronts; wave ly owns rows ly, ly+4, ly+8, ly+12
+
+struct CorrArgs {
+    Dims d;
+    const float *U, *V, *Wd, *G, *dz, *invG, *invGz;
+    const SpecPass *sp;
+    int nspec, tiles_x, nwork;
+};
+
+template <bool FCT>
+__global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
+{
+    // XCD-aware mapping: consecutive work items (the species of one tile)
+    // land on the same XCD (blocks are dealt round-robin over 8 XCDs).
+    const int nb = gridDim.x;
+    const int w = (blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3);
+    if (w >= a.nwork) return;
+    const int s = w % a.nspec;
+    const int tile = w / a.nspec;
+    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    const int nx = a.d.nx, ny = a.d.ny, nz = a.d.nz;
+    const size_t sk = (size_t)nx, sj = (size_t)nx * nz;
+    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    const int gi = tx * CTI - CH + lx;
+    const int j0 = ty * CTJ - CH;  // global row of staged row 0
+    const SpecPass p = a.sp[s];
+    const bool same = p.q1 == nullptr;
+    const float eps = 1e-10f, eps15 = 1e-15f;
+
+    __shared__ float sq1[4][CY][CX];  // q1 at levels m-3..m (ring by level & 3)
+    __shared__ float sq0[3][CY][CX];  // q0 at levels m-2..m (ring by level % 3)
+    __shared__ float sxu[CY][CX], sxf[CY][CX];  // x face left of a column: U2, donor flux
+    __shared__ float syu[CY][CX], syf[CY][CX];  // y face south of a row
+    __shared__ float sbxi[CY][CX], sbxo[CY][CX], sbyi[CY][CX], sbyo[CY][CX];  // FCT betas
+
+    const bool lane_in = gi >= 0 && gi < nx;
+    const bool lane_out = lx >= CH && lx < CX - CH && lane_in;
+    const int rb = (ly < 2) ? 1 : 0;  // own interior rows: ly + 4*(rb + t), t = 0..2
+
+    // per owned column (3 interior rows): z pipeline state
+    float z0a[3], z0b[3], z0c[3];        // q0 at levels m-2, m-1, m
+    float wu2[3], wu1[3], fz2[3], fz1[3];  // unlimited w / z donor flux at faces m-2, m-1
+    float bzi[3], bzo[3];                // betas of the cell below the current one
+    float Fzp[3];                        // limited z flux on the face below the output level
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        z0a[t] = z0b[t] = z0c[t] = 0.0f;
+        wu2[t] = wu1[t] = fz2[t] = fz1[t] = 0.0f;
+        bzi[t] = bzo[t] = 1.0f;
+        Fzp[t] = 0.0f;
+    }
+
+    for (int m = 0; m <= nz + 1; m++) {
+        const int L1 = m - 1, L2 = m - 2;
+        // ---------------- (A) stage level m
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = ly + 4 * r;
+            const int gj = j0 + row;
+            float v0 = 0.0f, v1 = 0.0f;
+            if (m < nz) {
+                if (lane_in && gj >= 0 && gj < ny) {
+                    const size_t c = (size_t)gi + sk * ((size_t)m + (size_t)nz * gj);
+                    v0 = ld_q(p.q0, c, p.kind0, p.scale0);
+                    v1 = same ? v0 : p.q1[c];
+                }
+                sq0[m % 3][row][lx] = v0;
+                sq1[m & 3][row][lx] = v1;
+            }
+            if (r >= rb && r < rb + 3) {  // own column: q0 at levels m-2, m-1, m
+                const int t = r - rb;
+                z0a[t] = z0b[t];
+                z0b[t] = z0c[t];
+                z0c[t] = v0;
+            }
+        }
+        __syncthreads();
+
+        // ---------------- (B) unlimited faces
+        // z face on top of level L1 for the owned columns (mpdata_fluxes :211-246)
+        if (L1 >= 0 && L1 <= nz - 1) {
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                const int row = ly + 4 * (rb + t);
+                const int gj = j0 + row;
+                float wv = 0.0f, fv = 0.0f;
+                if (lane_out && gj >= 0 && gj < ny && L1 <= nz - 2) {
+                    const size_t c = (size_t)gi + sk * ((size_t)L1 + (size_t)nz * gj);
+                    const float(*qa)[CX] = sq1[L1 & 3];
+                    const float(*qb)[CX] = sq1[m & 3];
+                    const float l = qa[row][lx], r = qb[row][lx];
+                    const float Wc = a.Wd[c];
+                    const float rG = rcp(a.G[c + sk] + a.G[c]);
+                    const float aw = fabsf(Wc);
+                    float val = aw * (1.0f - 2.0f * aw * rG) * ((r - l) * rcp(r + l + eps));
+                    float cross = 0.0f;
+                    if (gi >= 1 && gi <= nx - 2) {  // WxU (:227-233)
+                        const float sE = qa[row][lx + 1] + qb[row][lx + 1];
+                        const float sW = qa[row][lx - 1] + qb[row][lx - 1];
+                        const float ev = 0.25f * (a.U[c + 1] + a.U[c + 1 + sk] + a.U[c] + a.U[c + sk]);
+                        cross += ev * ((sE - sW) * rcp(sE + sW + eps));
+                    }
+                    if (gj >= 1 && gj <= ny - 2) {  // WxV (:238-245)
+                        const float sN = qa[row + 1][lx] + qb[row + 1][lx];
+                        const float sS = qa[row - 1][lx] + qb[row - 1][lx];
+                        const float ev = 0.25f * (a.V[c] + a.V[c + sk] + a.V[c + sj] + a.V[c + sk + sj]);
+                        cross += ev * ((sN - sS) * rcp(sN + sS + eps));
+                    }
+                    val -= 0.5f * Wc * rG * cross;
+                    wv = val * 0.5f * a.dz[c];  // :380
+                    fv = dflux(l, r, wv);
+                }
+                wu2[t] = wu1[t];
+                fz2[t] = fz1[t];
+                wu1[t] = wv;
+                fz1[t] = fv;
+            }
+        }
+        if (L2 >= 0) {
+            const float(*q1m)[CX] = sq1[L2 & 3];
+            const float(*q1u)[CX] = sq1[(L2 + 1) & 3];
+            const float(*q1d)[CX] = sq1[(L2 + 3) & 3];
+            const bool kin = L2 >= 1 && L2 <= nz - 2;
+            // x faces (left of column gi) on the owned rows, lanes 1..63 (:131-165)
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                const int row = ly + 4 * (rb + t);
+                const int gj = j0 + row;
+                float uv = 0.0f, fv = 0.0f;
+                if (lx >= 1 && gi >= 1 && gi <= nx - 1 && gj >= 0 && gj < ny) {
+                    const size_t c = (size_t)gi + sk * ((size_t)L2 + (size_t)nz * gj);
+                    const float r = q1m[row][lx], l = q1m[row][lx - 1];
+                    const float Uc = a.U[c];
+                    const float rG = rcp(a.G[c] + a.G[c - 1]);
+                    const float au = fabsf(Uc);
+                    float val = au * (1.0f - 2.0f * au * rG) * ((r - l) * rcp(r + l + eps));
+                    float cross = 0.0f;
+                    if (gj >= 1 && gj <= ny - 2) {  // UxV (:145-152)
+                        const float sN = q1m[row + 1][lx] + q1m[row + 1][lx - 1];
+                        const float sS = q1m[row - 1][lx] + q1m[row - 1][lx - 1];
+                        const float ev = 0.25f * (a.V[c] + a.V[c + sj] + a.V[c - 1] + a.V[c - 1 + sj]);
+                        cross += ev * ((sN - sS) * rcp(sN + sS + eps));
+                    }
+                    if (kin) {  // UxW (:157-164)
+                        const float sU = q1u[row][lx] + q1u[row][lx - 1];
+                        const float sD = q1d[row][lx] + q1d[row][lx - 1];
+                        const float ev = 0.25f * (a.Wd[c] + a.Wd[c - sk] + a.Wd[c - 1] + a.Wd[c - 1 - sk]);
+                        cross += ev * ((sU - sD) * rcp(sU + sD + eps));
+                    }
+                    val -= 0.5f * Uc * rG * cross;
+                    uv = val * 0.5f;  // :378
+                    fv = dflux(l, r, uv);
+                }
+                sxu[row][lx] = uv;
+                sxf[row][lx] = fv;
+            }
+            // y faces (south of row gj) on rows 1..15, lanes 2..61 (:169-205)
+#pragma unroll
+            for (int r4 = 0; r4 < 4; r4++) {
+                const int row = ly + 4 * r4;
+                if (row < 1) continue;
+                const int gj = j0 + row;
+                float vv = 0.0f, fv = 0.0f;
+                if (lane_out && gj >= 1 && gj <= ny - 1) {
+                    const size_t c = (size_t)gi + sk * ((size_t)L2 + (size_t)nz * gj);
+                    const float r = q1m[row][lx], l = q1m[row - 1][lx];
+                    const float Vc = a.V[c];
+                    const float rG = rcp(a.G[c] + a.G[c - sj]);
+                    const float av = fabsf(Vc);
+                    float val = av * (1.0f - 2.0f * av * rG) * ((r - l) * rcp(r + l + eps));
+                    float cross = 0.0f;
+                    if (gi >= 1 && gi <= nx - 2) {  // VxU (:186-192)
+                        const float sE = q1m[row][lx + 1] + q1m[row - 1][lx + 1];
+                        const float sW = q1m[row][lx - 1] + q1m[row - 1][lx - 1];
+                        const float ev = 0.25f * (a.U[c + 1] + a.U[c + 1 - sj] + a.U[c] + a.U[c - sj]);
+                        cross += ev * ((sE - sW) * rcp(sE + sW + eps));
+                    }
+                    if (kin) {  // VxW (:197-204)
+                        const float sU = q1u[row][lx] + q1u[row - 1][lx];
+                        const float sD = q1d[row][lx] + q1d[row - 1][lx];
+                        const float ev = 0.25f * (a.Wd[c] + a.Wd[c - sk] + a.Wd[c - sj] + a.Wd[c - sk - sj]);
+                        cross += ev * ((sU - sD) * rcp(sU + sD + eps));
+                    }
+                    val -= 0.5f * Vc * rG * cross;
+                    vv = val * 0.5f;  // :379
+                    fv = dflux(l, r, vv);
+                }
+                syu[row][lx] = vv;
+                syf[row][lx] = fv;
+            }
+        }
+        __syncthreads();
+
+        // ---------------- (C) FCT: per-cell betas (adv_mpdata_FCT_core.f90:47-116)
+        float Fzc[3] = {0.0f, 0.0f, 0.0f};  // limited z flux on top of level L2
+        if (FCT) {
+            if (L2 >= 0) {
+                const float(*q1m)[CX] = sq1[L2 & 3];
+                const float(*q0m)[CX] = sq0[L2 % 3];
+                // x cells of the owned rows, lanes 1..62
+#pragma unroll
+                for (int t = 0; t < 3; t++) {
+                    const int row = ly + 4 * (rb + t);
+                    if (lx >= 1 && lx <= CX - 2 && lane_in) {
+                        float qmax, qmin, fin = 0.0f, fout = 0.0f;
+                        const float qc1 = q1m[row][lx];
+                        if (gi == 0) {
+                            qmax = fmaxf(fmaxf(qc1, q1m[row][lx + 1]), fmaxf(q0m[row][lx], q0m[row][lx + 1]));
+                            qmin = fminf(fminf(qc1, q1m[row][lx + 1]), fminf(q0m[row][lx], q0m[row][lx + 1]));
+                        } else if (gi == nx - 1) {
+                            qmax = max3(q1m[row][lx - 1], qc1, q0m[row][lx - 1]);
+                            qmin = min3(q1m[row][lx - 1], qc1, q0m[row][lx - 1]);
+                        } else {
+                            qmax = fmaxf(max3(q1m[row][lx - 1], qc1, q1m[row][lx + 1]),
+                                         max3(q0m[row][lx - 1], q0m[row][lx], q0m[row][lx + 1]));
+                            qmin = fminf(min3(q1m[row][lx - 1], qc1, q1m[row][lx + 1]),
+                                         min3(q0m[row][lx - 1], q0m[row][lx], q0m[row][lx + 1]));
+                            const float fl = sxf[row][lx], fr = sxf[row][lx + 1];
+                            fin = fmaxf(0.0f, fl) - fminf(0.0f, fr);
+                            fout = fmaxf(0.0f, fr) - fminf(0.0f, fl);
+                        }
+                        sbxi[row][lx] = (qmax - qc1) * rcp(fin + eps15);
+                        sbxo[row][lx] = (qc1 - qmin) * rcp(fout + eps15);
+                    }
+                }
+                // y cells on rows 1..14, lanes 2..61
+#pragma unroll
+                for (int r4 = 0; r4 < 4; r4++) {
+                    const int row = ly + 4 * r4;
+                    if (row < 1 || row > CY - 2) continue;
+                    const int gj = j0 + row;
+                    if (lane_out && gj >= 0 && gj < ny) {
+                        float qmax, qmin, fin = 0.0f, fout = 0.0f;
+                        const float qc1 = q1m[row][lx];
+                        if (gj == 0) {
+                            qmax = fmaxf(fmaxf(qc1, q1m[row + 1][lx]), fmaxf(q0m[row][lx], q0m[row + 1][lx]));
+                            qmin = fminf(fminf(qc1, q1m[row + 1][lx]), fminf(q0m[row][lx], q0m[row + 1][lx]));
+                        } else if (gj == ny - 1) {
+                            qmax = max3(q1m[row - 1][lx], qc1, q0m[row - 1][lx]);
+                            qmin = min3(q1m[row - 1][lx], qc1, q0m[row - 1][lx]);
+                        } else {
+                            qmax = fmaxf(max3(q1m[row - 1][lx], qc1, q1m[row + 1][lx]),
+                                         max3(q0m[row - 1][lx], q0m[row][lx], q0m[row + 1][lx]));
+                            qmin = fminf(min3(q1m[row - 1][lx], qc1, q1m[row + 1][lx]),
+                                         min3(q0m[row - 1][lx], q0m[row][lx], q0m[row + 1][lx]));
+                            const float fs = syf[row][lx], fn = syf[row + 1][lx];
+                            fin = fmaxf(0.0f, fs) - fminf(0.0f, fn);
+                            fout = fmaxf(0.0f, fn) - fminf(0.0f, fs);
+                        }
+                        sbyi[row][lx] = (qmax - qc1) * rcp(fin + eps15);
+                        sbyo[row][lx] = (qc1 - qmin) * rcp(fout + eps15);
+                    }
+                }
+            }
+            // z: betas of cell L1 of each owned column, then the limited flux
+            // on the face between L2 and L1 (flux_is_w end rules)
+            if (L1 >= 0 && L1 <= nz - 1) {
+#pragma unroll
+                for (int t = 0; t < 3; t++) {
+                    const int row = ly + 4 * (rb + t);
+                    const float qc1 = sq1[L1 & 3][row][lx];
+                    float qmax, qmin, fin, fout;
+                    if (L1 == 0) {
+                        const float qu1 = sq1[m & 3][row][lx];
+                        qmax = fmaxf(fmaxf(qc1, qu1), fmaxf(z0b[t], z0c[t]));
+                        qmin = fminf(fminf(qc1, qu1), fminf(z0b[t], z0c[t]));
+                        fin = 0.0f - fminf(0.0f, fz1[t]);
+                        fout = fmaxf(0.0f, fz1[t]);
+                    } else if (L1 == nz - 1) {
+                        const float qd1 = sq1[L2 & 3][row][lx];
+                        qmax = max3(qd1, qc1, z0a[t]);
+                        qmin = min3(qd1, qc1, z0a[t]);
+                        fin = fmaxf(0.0f, fz2[t]) - fminf(0.0f, fz2[t]);
+                        fout = fin;
+                    } else {
+                        const float qd1 = sq1[L2 & 3][row][lx], qu1 = sq1[m & 3][row][lx];
+                        qmax = fmaxf(max3(qd1, qc1, qu1), max3(z0a[t], z0b[t], z0c[t]));
+                        qmin = fminf(min3(qd1, qc1, qu1), min3(z0a[t], z0b[t], z0c[t]));
+                        fin = fmaxf(0.0f, fz2[t]) - fminf(0.0f, fz1[t]);
+                        fout = fmaxf(0.0f, fz1[t]) - fminf(0.0f, fz2[t]);
+                    }
+                    const float bi = (qmax - qc1) * rcp(fin + eps15);
+                    const float bo = (qc1 - qmin) * rcp(fout + eps15);
+                    if (L2 >= 0 && L2 <= nz - 2) {
+                        const float u = wu2[t];
+                        float fac = 0.0f;
+                        if (u > 0.0f)
+                            fac = fminf(1.0f, fminf(bi, bzo[t]));
+                        else if (u < 0.0f)
+                            fac = fminf(1.0f, fminf(bzi[t], bo));
+                        Fzc[t] = fac * fz2[t];
+                    }
+                    bzi[t] = bi;
+                    bzo[t] = bo;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < 3; t++) Fzc[t] = (L2 >= 0 && L2 <= nz - 2) ? fz2[t] : 0.0f;
+        }
+        __syncthreads();
+
+        // ---------------- (D) limited x/y donor fluxes (in place of the unlimited ones)
+        if (FCT && L2 >= 0) {
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                const int row = ly + 4 * (rb + t);
+                if (lx >= 2 && lx <= CX - 2) {
+                    const float u = sxu[row][lx];
+                    float fac = 0.0f;
+                    if (u > 0.0f)
+                        fac = fminf(1.0f, fminf(sbxi[row][lx], sbxo[row][lx - 1]));
+                    else if (u < 0.0f)
+                        fac = fminf(1.0f, fminf(sbxi[row][lx - 1], sbxo[row][lx]));
+                    sxf[row][lx] = fac * sxf[row][lx];
+                }
+            }
+#pragma unroll
+            for (int r4 = 0; r4 < 4; r4++) {
+                const int row = ly + 4 * r4;
+                if (row < 2 || row > CY - 2) continue;
+                if (lane_out) {
+                    const float u = syu[row][lx];
+                    float fac = 0.0f;
+                    if (u > 0.0f)
+                        fac = fminf(1.0f, fminf(sbyi[row][lx], sbyo[row - 1][lx]));
+                    else if (u < 0.0f)
+                        fac = fminf(1.0f, fminf(sbyi[row - 1][lx], sbyo[row][lx]));
+                    syf[row][lx] = fac * syf[row][lx];
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---------------- (E) flux divergence on level L2 (upwind_advection with U2)
+        if (L2 >= 0) {
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                const int row = ly + 4 * (rb + t);
+                const int gj = j0 + row;
+                if (lane_out && gj >= 0 && gj < ny) {
+                    const size_t c = (size_t)gi + sk * ((size_t)L2 + (size_t)nz * gj);
+                    const float qc1 = sq1[L2 & 3][row][lx];
+                    float q = qc1;
+                    if (!(gi == 0 || gi == nx - 1 || gj == 0 || gj == ny - 1)) {
+                        const float H = (sxf[row][lx + 1] - sxf[row][lx]) + (syf[row + 1][lx] - syf[row][lx]);
+                        q = q - H * a.invG[c];
+                        q = q - (Fzc[t] - Fzp[t]) * a.invGz[c];
+                    }
+                    st_q(p.out, c, p.kindo, p.inv_scaleo, q);
+                }
+                Fzp[t] = Fzc[t];
+            }
+        }
+    }
+}
+
+unsigned grid_for(size_t n)
+{
+    size_t b = (n + 255) / 256;
+    if (b > 16384) b = 16384;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct FastState {
+    Dims d{};
+    size_t N = 0;
+    std::string err;
+    std::vector<FastSpecies> sp;      // home buffers as bound
+    std::vector<void *> alt;          // second home buffer per species
+    int cur = 0;                      // which home holds the current state
+    float *s1 = nullptr, *s2 = nullptr;  // FP32 scratch, nspec * N each
+    float *invG = nullptr, *invGz = nullptr;
+    SpecPass *d_tab = nullptr;        // [2 parities][max passes][nspec]
+    int tab_order = -1;
+    size_t tab_cap = 0;
+};
+
+namespace {
+
+size_t kind_bytes(int kind) { return kind == ICAR_ADV_STORE_F32 ? 4 : 2; }
+
+void free_species_buffers(FastState *st)
+{
+    for (void *p : st->alt) (void)hipFree(p);
+    st->alt.clear();
+    (void)hipFree(st->s1);
+    (void)hipFree(st->s2);
+    st->s1 = st->s2 = nullptr;
+}
+
+void *home(FastState *st, int s, int which)
+{
+    return which == 0 ? st->sp[s].dev : st->alt[s];
+}
+
+// Builds the per-pass species tables for both home parities.
+int build_tables(FastState *st, int order)
+{
+    const int S = (int)st->sp.size();
+    const int P = order < 1 ? 1 : order;
+    const size_t need = (size_t)2 * P * S;
+    if (need > st->tab_cap) {
+        (void)hipFree(st->d_tab);
+        st->d_tab = nullptr;
+        if (hipMalloc(&st->d_tab, need * sizeof(SpecPass)) != hipSuccess) {
+            st->err = "hipMalloc of the species pass tables failed";
+            st->tab_cap = 0;
+            return ICAR_ADV_ENOMEM;
+        }
+        st->tab_cap = need;
+    }
+    std::vector<SpecPass> h(need);
+    for (int par = 0; par < 2; par++)
+        for (int pass = 1; pass <= P; pass++)
+            for (int s = 0; s < S; s++) {
+                const FastSpecies &f = st->sp[s];
+                const float sc = ldexpf(1.0f, f.pow2), isc = ldexpf(1.0f, -f.pow2);
+                float *sa = st->s1 + (size_t)s * st->N, *sb = st->s2 + (size_t)s * st->N;
+                SpecPass e{};
+                void *hcur = home(st, s, par), *hoth = home(st, s, 1 - par);
+                // input of this pass
+                if (pass == 1) {
+                    e.q0 = hcur, e.kind0 = f.storage, e.scale0 = sc, e.q1 = nullptr;
+                } else if (pass == 2) {
+                    e.q0 = hcur, e.kind0 = f.storage, e.scale0 = sc, e.q1 = sa;
+                } else {
+                    e.q0 = (pass % 2 == 1) ? sb : sa;  // output of the previous pass
+                    e.kind0 = ICAR_ADV_STORE_F32, e.scale0 = 1.0f, e.q1 = nullptr;
+                }
+                // output of this pass
+                if (pass == P) {
+                    e.out = (P <= 2) ? hoth : hcur;
+                    e.kindo = f.storage, e.inv_scaleo = isc;
+                } else {
+                    e.out = (pass % 2 == 1) ? sa : sb;
+                    e.kindo = ICAR_ADV_STORE_F32, e.inv_scaleo = 1.0f;
+                }
+                h[((size_t)par * P + (pass - 1)) * S + s] = e;
+            }
+    if (hipMemcpy(st->d_tab, h.data(), need * sizeof(SpecPass), hipMemcpyHostToDevice) !=
+        hipSuccess) {
+        st->err = "upload of the species pass tables failed";
+        return ICAR_ADV_ENOMEM;
+    }
+    st->tab_order = order;
+    return 0;
+}
+
+}  // namespace
 
 FastState *fast_create(Dims d)
 {
     FastState *st = new FastState();
     st->d = d;
+    st->N = (size_t)d.nx * d.ny * d.nz;
+    if (hipMalloc(&st->invG, st->N * sizeof(float)) != hipSuccess ||
+        hipMalloc(&st->invGz, st->N * sizeof(float)) != hipSuccess) {
+        fast_destroy(st);
+        return nullptr;
+    }
     return st;
 }
 
-void fast_destroy(FastState *st) { delete st; }
+void fast_destroy(FastState *st)
+{
+    if (!st) return;
+    free_species_buffers(st);
+    (void)hipFree(st->invG);
+    (void)hipFree(st->invGz);
+    (void)hipFree(st->d_tab);
+    delete st;
+}
 
-int fast_set_species(FastState *, const FastSpecies *, int) { return 0; }
+int fast_set_species(FastState *st, const FastSpecies *sp, int n)
+{
+    free_species_buffers(st);
+    st->sp.assign(sp, sp + n);
+    st->cur = 0;
+    st->tab_order = -1;
+    if (n == 0) return 0;
+    st->alt.assign(n, nullptr);
+    for (int s = 0; s < n; s++)
+        if (hipMalloc(&st->alt[s], st->N * kind_bytes(sp[s].storage)) != hipSuccess) {
+            st->err = "hipMalloc of a species buffer failed";
+            return ICAR_ADV_ENOMEM;
+        }
+    if (hipMalloc(&st->s1, (size_t)n * st->N * sizeof(float)) != hipSuccess ||
+        hipMalloc(&st->s2, (size_t)n * st->N * sizeof(float)) != hipSuccess) {
+        st->err = "hipMalloc of the FP32 pass scratch failed";
+        return ICAR_ADV_ENOMEM;
+    }
+    return 0;
+}
+
+void *fast_species_ptr(FastState *st, int s)
+{
+    if (!st || s < 0 || s >= (int)st->sp.size()) return nullptr;
+    return home(st, s, st->cur);
+}
 
 const char *fast_error(FastState *st) { return st ? st->err.c_str() : "no fused state"; }
 
-int fast_advect(FastState *st, hipStream_t, Dims, const StepFields &, const DomainFields &, int, int,
-                const TimedFn &)
+int fast_advect(FastState *st, hipStream_t s, Dims d, const StepFields &sf, const DomainFields &f,
+                int order, int fct, const TimedFn &timed)
 {
-    st->err = "kernel_variant 1 (fused path) is not available in this build";
-    return ICAR_ADV_EINVAL;
+    const int S = (int)st->sp.size();
+    if (S == 0 || order < 1) return 0;
+    if (st->tab_order != order) {
+        int rc = build_tables(st, order);
+        if (rc) return rc;
+    }
+    const int P = order;
+    const double n = (double)st->N;
+    const SpecPass *tab = st->d_tab + (size_t)st->cur * P * S;
+    timed(K_CONVERT, 4.0 * 4 * n, [&] {
+        hipLaunchKernelGGL(k_inv, dim3(grid_for(st->N)), dim3(256), 0, s, st->N, sf.G, sf.Gz,
+                           st->invG, st->invGz);
+    });
+    // pass 1: donor cell for every species
+    timed(K_FUSED_FIRST, (5.0 * 4 + S * 2.0 * 4) * n, [&] {
+        dim3 grid((d.nx + FBX - 1) / FBX, d.nz, (d.ny + FBY - 1) / FBY);
+        hipLaunchKernelGGL(k_first, grid, dim3(FBX, FBY), 0, s, d, sf.U, sf.V, sf.W, st->invG,
+                           st->invGz, tab, S);
+    });
+    // corrective passes
+    CorrArgs a;
+    a.d = d;
+    a.U = sf.U;
+    a.V = sf.V;
+    a.Wd = sf.Wd;
+    a.G = sf.G;
+    a.dz = f.dz;
+    a.invG = st->invG;
+    a.invGz = st->invGz;
+    a.nspec = S;
+    a.tiles_x = (d.nx + CTI - 1) / CTI;
+    const int tiles_y = (d.ny + CTJ - 1) / CTJ;
+    a.nwork = a.tiles_x * tiles_y * S;
+    const int nb = (a.nwork + 7) / 8 * 8;
+    for (int pass = 2; pass <= P; pass++) {
+        a.sp = tab + (size_t)(pass - 1) * S;
+        // species read (q0, q1 at pass 2) and written once, invariants once
+        const double bytes = (pass == 2 ? 3.0 : 2.0) * 4 * n * S + 6.0 * 4 * n;
+        timed(K_FUSED_CORR, bytes, [&] {
+            if (fct)
+                hipLaunchKernelGGL(k_corr<true>, dim3(nb), dim3(CNT), 0, s, a);
+            else
+                hipLaunchKernelGGL(k_corr<false>, dim3(nb), dim3(CNT), 0, s, a);
+        });
+    }
+    if (P <= 2) st->cur = 1 - st->cur;  // the result landed in the other home buffer
+    return 0;
 }
+
+void launch_pack(hipStream_t s, size_t n, const float *src, void *dst, int storage, int pow2);
+void launch_unpack(hipStream_t s, size_t n, const void *src, float *dst, int storage, int pow2);
 
 namespace {
-
-__device__ __forceinline__ float load16(const uint16_t *p, size_t i, int storage, float scale)
-{
-    const uint16_t b = p[i];
-    float v;
-    if (storage == ICAR_ADV_STORE_BF16)
-        v = __uint_as_float((uint32_t)b << 16);
-    else
-        v = __half2float(__ushort_as_half(b));
-    return v * scale;
-}
-
-__device__ __forceinline__ uint16_t store16(float v, int storage, float inv_scale)
-{
-    const float x = v * inv_scale;
-    if (storage == ICAR_ADV_STORE_BF16) {
-        // round to nearest even
-        uint32_t u = __float_as_uint(x);
-        if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
-        u += 0x7fffu + ((u >> 16) & 1u);
-        return (uint16_t)(u >> 16);
-    }
-    return __half_as_ushort(__float2half_rn(x));
-}
 
 __global__ void k_pack(size_t n, const float *__restrict__ src, uint16_t *__restrict__ dst,
                        int storage, float inv_scale)
 {
     for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
          p += (size_t)gridDim.x * blockDim.x)
-        dst[p] = store16(src[p], storage, inv_scale);
+        dst[p] = to16(src[p], storage, inv_scale);
 }
 
 __global__ void k_unpack(size_t n, const uint16_t *__restrict__ src, float *__restrict__ dst,
@@ -74,7 +738,7 @@ __global__ void k_unpack(size_t n, const uint16_t *__restrict__ src, float *__re
 {
     for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
          p += (size_t)gridDim.x * blockDim.x)
-        dst[p] = load16(src, p, storage, scale);
+        dst[p] = ld_q(src, p, storage, scale);
 }
 
 __global__ void k_bc16_x(Dims d, uint16_t *q, int bc_x)
@@ -97,13 +761,6 @@ __global__ void k_bc16_y(Dims d, uint16_t *q)
     if (i >= d.nx || k >= d.nz) return;
     q[ix(d, i, k, 0)] = q[ix(d, i, k, d.ny - 2)];
     q[ix(d, i, k, d.ny - 1)] = q[ix(d, i, k, 1)];
-}
-
-unsigned grid_for(size_t n)
-{
-    size_t b = (n + 255) / 256;
-    if (b > 16384) b = 16384;
-    return (unsigned)(b < 1 ? 1 : b);
 }
 
 }  // namespace
