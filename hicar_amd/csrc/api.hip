@@ -702,7 +702,7 @@ const char *kernel_name(int id)
     static const char *names[K_COUNT] = {"courant",     "upwind",     "pseudo_velocity",
                                          "fct",         "ring_bc",    "reduce",
                                          "copy",        "fused_first_pass", "fused_corrective_pass",
-                                         "convert16"};
+                                         "convert16",   "face_coefficients"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "";
 }
 }  // namespace icaradv

@@ -35,6 +35,7 @@ enum KernelId {
     K_FUSED_FIRST,
     K_FUSED_CORR,
     K_CONVERT,
+    K_COEF,
     K_COUNT
 };
 

@@ -1,10 +1,12 @@
 // Fused, species-batched MPDATA path (kernel_variant 1).
 //
 // One advect() step of mpdata_order n (adv_mpdata.f90:352-413) becomes
-//   k_inv      1/(jaco*rho) and 1/(dz*jaco*rho) once per step
+//   k_coef     species-invariant face coefficients of the pseudo-velocities
+//              and the divergence reciprocals, once per step
 //   k_first    the donor-cell pass (upwind_advection, :43-103) for every
-//              species in one launch, the Courant numbers and reciprocals
-//              loaded once per cell and reused across species
+//              species in one launch, the Courant numbers loaded once per
+//              cell and reused across species (bit-identical to the
+//              reference order)
 //   k_corr     one launch per corrective pass (iord >= 2) for every species:
 //              antidiffusive pseudo-velocities (mpdata_fluxes, :105-252),
 //              the 1-D FCT limiter in x, y and z (flux_limiter, :254-350 and
@@ -83,28 +85,84 @@ struct SpecPass {
 };
 
 // ---------------------------------------------------------------------------
-// reciprocals of the divergence denominators, once per step
+// species-invariant face coefficients, once per step (mpdata_fluxes
+// :139-151,163,180-203,221-244 with the damping of :378-380 folded in):
+//   u2 = cxa*A - cxv*A_y - cxw*A_z        (x face left of the cell)
+//   v2 = cya*A - cyu*A_x - cyw*A_z        (y face south of the cell)
+//   w2 = cza*A - czu*A_x - czv*A_y        (z face on top of the cell)
+// with A the face ratio (r-l)/(r+l+1e-10) and A_* the four-cell cross
+// ratios of the species.  Coefficients of terms the reference skips (its
+// row/level guards, missing faces) are 0.  Planes 9/10 hold 1/(jaco*rho)
+// and 1/(dz*jaco*rho) for the flux divergence.
 // ---------------------------------------------------------------------------
-__global__ void k_inv(size_t n, const float *__restrict__ G, const float *__restrict__ Gz,
-                      float *__restrict__ invG, float *__restrict__ invGz)
+enum { CXA = 0, CXV, CXW, CYA, CYU, CYW, CZA, CZU, CZV, CIG, CIGZ, NCOEF };
+
+__global__ void __launch_bounds__(256)
+    k_coef(Dims d, const float *__restrict__ U, const float *__restrict__ V,
+           const float *__restrict__ Wd, const float *__restrict__ G, const float *__restrict__ Gz,
+           const float *__restrict__ dz, float *__restrict__ coef)
 {
-    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-         p += (size_t)gridDim.x * blockDim.x) {
-        invG[p] = 1.0f / G[p];
-        invGz[p] = 1.0f / Gz[p];
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.z * 4 + threadIdx.y;
+    const int k = blockIdx.y;
+    if (i >= d.nx || j >= d.ny) return;
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const size_t N = (size_t)nx * ny * nz;
+    const size_t sk = (size_t)nx, sj = (size_t)nx * nz;
+    const size_t c = (size_t)i + sk * ((size_t)k + (size_t)nz * j);
+    float v[NCOEF];
+#pragma unroll
+    for (int e = 0; e < NCOEF; e++) v[e] = 0.0f;
+    if (i >= 1) {
+        const float Uc = U[c], sG = G[c] + G[c - 1], au = fabsf(Uc);
+        v[CXA] = 0.5f * au * (1.0f - 2.0f * au / sG);
+        if (j >= 1 && j <= ny - 2)
+            v[CXV] = 0.25f * Uc * (0.25f * (V[c] + V[c + sj] + V[c - 1] + V[c - 1 + sj])) / sG;
+        if (k >= 1 && k <= nz - 2)
+            v[CXW] = 0.25f * Uc * (0.25f * (Wd[c] + Wd[c - sk] + Wd[c - 1] + Wd[c - 1 - sk])) / sG;
     }
+    if (j >= 1) {
+        const float Vc = V[c], sG = G[c] + G[c - sj], av = fabsf(Vc);
+        v[CYA] = 0.5f * av * (1.0f - 2.0f * av / sG);
+        if (i >= 1 && i <= nx - 2)
+            v[CYU] = 0.25f * Vc * (0.25f * (U[c + 1] + U[c + 1 - sj] + U[c] + U[c - sj])) / sG;
+        if (k >= 1 && k <= nz - 2)
+            v[CYW] = 0.25f * Vc * (0.25f * (Wd[c] + Wd[c - sk] + Wd[c - sj] + Wd[c - sk - sj])) / sG;
+    }
+    if (k <= nz - 2) {
+        const float Wc = Wd[c], sG = G[c + sk] + G[c], aw = fabsf(Wc), h = 0.5f * dz[c];
+        v[CZA] = h * aw * (1.0f - 2.0f * aw / sG);
+        if (i >= 1 && i <= nx - 2)
+            v[CZU] = h * 0.5f * Wc * (0.25f * (U[c + 1] + U[c + 1 + sk] + U[c] + U[c + sk])) / sG;
+        if (j >= 1 && j <= ny - 2)
+            v[CZV] = h * 0.5f * Wc * (0.25f * (V[c] + V[c + sk] + V[c + sj] + V[c + sk + sj])) / sG;
+    }
+    v[CIG] = 1.0f / G[c];
+    v[CIGZ] = 1.0f / Gz[c];
+#pragma unroll
+    for (int e = 0; e < NCOEF; e++) coef[(size_t)e * N + c] = v[e];
 }
 
 // ---------------------------------------------------------------------------
 // donor-cell pass for every species (upwind_advection, adv_mpdata.f90:43-103)
+// in the reference's operation order with IEEE division: its result is
+// bit-identical to the exact kernel's, so the first-pass field and the FCT
+// bounds built from it match the reference exactly.
 // ---------------------------------------------------------------------------
 constexpr int FBX = 64, FBY = 4;
 
+__device__ __forceinline__ float xflux1(float l, float r, float U)
+{
+#pragma clang fp contract(off)
+    return ((U + fabsf(U)) * l + (U - fabsf(U)) * r) / 2.0f;
+}
+
 __global__ void __launch_bounds__(FBX *FBY)
     k_first(Dims d, const float *__restrict__ U, const float *__restrict__ V,
-            const float *__restrict__ W, const float *__restrict__ invG,
-            const float *__restrict__ invGz, const SpecPass *__restrict__ sp, int nspec)
+            const float *__restrict__ W, const float *__restrict__ G,
+            const float *__restrict__ Gz, const SpecPass *__restrict__ sp, int nspec)
 {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * FBX + threadIdx.x;
     const int j = blockIdx.z * FBY + threadIdx.y;
     const int k = blockIdx.y;
@@ -112,7 +170,7 @@ __global__ void __launch_bounds__(FBX *FBY)
     const size_t sk = (size_t)d.nx, sj = (size_t)d.nx * d.nz;
     const size_t c = (size_t)i + sk * ((size_t)k + (size_t)d.nz * j);
     const bool ring = i == 0 || i == d.nx - 1 || j == 0 || j == d.ny - 1;
-    float Ul = 0, Ur = 0, Vs = 0, Vn = 0, Wt = 0, Wb = 0, iG = 0, iGz = 0;
+    float Ul = 0, Ur = 0, Vs = 0, Vn = 0, Wt = 0, Wb = 0, g = 1, gz = 1;
     if (!ring) {
         Ul = U[c];
         Ur = U[c + 1];
@@ -120,8 +178,8 @@ __global__ void __launch_bounds__(FBX *FBY)
         Vn = V[c + sj];
         Wt = W[c];
         Wb = k > 0 ? W[c - sk] : 0.0f;
-        iG = invG[c];
-        iGz = invGz[c];
+        g = G[c];
+        gz = Gz[c];
     }
     for (int s = 0; s < nspec; s++) {
         const SpecPass p = sp[s];
@@ -134,19 +192,19 @@ __global__ void __launch_bounds__(FBX *FBY)
         const float qe = ld_q(p.q0, c + 1, p.kind0, p.scale0);
         const float qs = ld_q(p.q0, c - sj, p.kind0, p.scale0);
         const float qn = ld_q(p.q0, c + sj, p.kind0, p.scale0);
-        const float fr = dflux(qc, qe, Ur), fl = dflux(qw, qc, Ul);
-        const float fn = dflux(qc, qn, Vn), fs = dflux(qs, qc, Vs);
-        float q = qc - ((fr - fl) + (fn - fs)) * iG;
+        const float fr = xflux1(qc, qe, Ur), fl = xflux1(qw, qc, Ul);
+        const float fn = xflux1(qc, qn, Vn), fs = xflux1(qs, qc, Vs);
+        float q = qc - ((fr - fl) + (fn - fs)) / g;  // :86-87
         if (k == 0) {
             const float qu = ld_q(p.q0, c + sk, p.kind0, p.scale0);
-            q = q - dflux(qc, qu, Wt) * iGz;
+            q = q - xflux1(qc, qu, Wt) / gz;  // :93-94
         } else if (k == d.nz - 1) {
             const float qd = ld_q(p.q0, c - sk, p.kind0, p.scale0);
-            q = q - (qc * Wt - dflux(qd, qc, Wb)) * iGz;  // top flux not upwinded (:96-97)
+            q = q - (qc * Wt - xflux1(qd, qc, Wb)) / gz;  // top flux not upwinded (:96-97)
         } else {
             const float qu = ld_q(p.q0, c + sk, p.kind0, p.scale0);
             const float qd = ld_q(p.q0, c - sk, p.kind0, p.scale0);
-            q = q - (dflux(qc, qu, Wt) - dflux(qd, qc, Wb)) * iGz;
+            q = q - (xflux1(qc, qu, Wt) - xflux1(qd, qc, Wb)) / gz;  // :90-91
         }
         st_q(p.out, c, p.kindo, p.inv_scaleo, q);
     }
@@ -164,16 +222,25 @@ constexpr int CNT = 256;         // 4 wavefronts; wave ly owns rows ly, ly+4, ly
 
 struct CorrArgs {
     Dims d;
-    const float *U, *V, *Wd, *G, *dz, *invG, *invGz;
+    const float *coef;  // NCOEF planes of N
     const SpecPass *sp;
     int nspec, tiles_x, nwork;
 };
+
+// four-cell cross ratio ((a-b)+(c-d)) / ((a+b)+(c+d)+1e-10)
+__device__ __forceinline__ float cross_ratio(float a, float b, float c, float d)
+{
+    return ((a - b) + (c - d)) * rcp(((a + b) + (c + d)) + 1e-10f);
+}
+
+__device__ __forceinline__ float face_ratio(float l, float r) { return (r - l) * rcp(r + l + 1e-10f); }
 
 template <bool FCT>
 __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
 {
     // XCD-aware mapping: consecutive work items (the species of one tile)
-    // land on the same XCD (blocks are dealt round-robin over 8 XCDs).
+    // land on the same XCD (blocks are dealt round-robin over 8 XCDs) and
+    // share the tile's coefficients in its L2.
     const int nb = gridDim.x;
     const int w = (blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3);
     if (w >= a.nwork) return;
@@ -181,13 +248,15 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
     const int tile = w / a.nspec;
     const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
     const int nx = a.d.nx, ny = a.d.ny, nz = a.d.nz;
+    const size_t N = (size_t)nx * ny * nz;
     const size_t sk = (size_t)nx, sj = (size_t)nx * nz;
     const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
     const int gi = tx * CTI - CH + lx;
     const int j0 = ty * CTJ - CH;  // global row of staged row 0
     const SpecPass p = a.sp[s];
     const bool same = p.q1 == nullptr;
-    const float eps = 1e-10f, eps15 = 1e-15f;
+    const float eps15 = 1e-15f;
+    const float *__restrict__ cf = a.coef;
 
     __shared__ float sq1[4][CY][CX];  // q1 at levels m-3..m (ring by level & 3)
     __shared__ float sq0[3][CY][CX];  // q0 at levels m-2..m (ring by level % 3)
@@ -199,11 +268,21 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
     const bool lane_out = lx >= CH && lx < CX - CH && lane_in;
     const int rb = (ly < 2) ? 1 : 0;  // own interior rows: ly + 4*(rb + t), t = 0..2
 
+    // own rows / staged rows in the domain
+    bool row_in[4];
+    size_t rowbase[4];  // gi + sj*gj (level offset added per use)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int gj = j0 + ly + 4 * r;
+        row_in[r] = lane_in && gj >= 0 && gj < ny;
+        rowbase[r] = row_in[r] ? (size_t)gi + sj * (size_t)gj : 0;
+    }
+
     // per owned column (3 interior rows): z pipeline state
-    float z0a[3], z0b[3], z0c[3];        // q0 at levels m-2, m-1, m
+    float z0a[3], z0b[3], z0c[3];          // q0 at levels m-2, m-1, m
     float wu2[3], wu1[3], fz2[3], fz1[3];  // unlimited w / z donor flux at faces m-2, m-1
-    float bzi[3], bzo[3];                // betas of the cell below the current one
-    float Fzp[3];                        // limited z flux on the face below the output level
+    float bzi[3], bzo[3];                  // betas of the cell below the current one
+    float Fzp[3];                          // limited z flux on the face below the output level
 #pragma unroll
     for (int t = 0; t < 3; t++) {
         z0a[t] = z0b[t] = z0c[t] = 0.0f;
@@ -212,64 +291,99 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
         Fzp[t] = 0.0f;
     }
 
+    // register prefetch of the next staged level
+    float n0[4], n1[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        n0[r] = n1[r] = 0.0f;
+        if (row_in[r]) {
+            n0[r] = ld_q(p.q0, rowbase[r], p.kind0, p.scale0);
+            n1[r] = same ? n0[r] : p.q1[rowbase[r]];
+        }
+    }
+
     for (int m = 0; m <= nz + 1; m++) {
         const int L1 = m - 1, L2 = m - 2;
-        // ---------------- (A) stage level m
+        // ---------------- (A) stage level m, prefetch level m+1 and this
+        //                  iteration's coefficients
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int row = ly + 4 * r;
-            const int gj = j0 + row;
-            float v0 = 0.0f, v1 = 0.0f;
             if (m < nz) {
-                if (lane_in && gj >= 0 && gj < ny) {
-                    const size_t c = (size_t)gi + sk * ((size_t)m + (size_t)nz * gj);
-                    v0 = ld_q(p.q0, c, p.kind0, p.scale0);
-                    v1 = same ? v0 : p.q1[c];
-                }
-                sq0[m % 3][row][lx] = v0;
-                sq1[m & 3][row][lx] = v1;
+                sq0[m % 3][row][lx] = n0[r];
+                sq1[m & 3][row][lx] = n1[r];
             }
             if (r >= rb && r < rb + 3) {  // own column: q0 at levels m-2, m-1, m
                 const int t = r - rb;
                 z0a[t] = z0b[t];
                 z0b[t] = z0c[t];
-                z0c[t] = v0;
+                z0c[t] = (m < nz) ? n0[r] : 0.0f;
+            }
+        }
+        if (m + 1 < nz) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                if (row_in[r]) {
+                    const size_t c = rowbase[r] + sk * (size_t)(m + 1);
+                    n0[r] = ld_q(p.q0, c, p.kind0, p.scale0);
+                    n1[r] = same ? n0[r] : p.q1[c];
+                }
+            }
+        }
+        // coefficients: x faces and divergence at L2 (own rows), y faces at
+        // L2 (rows 1..15), z faces at L1 (own rows)
+        float kx[3][3], kz[3][3], ky[4][3], kg[3][2];
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            const int r = rb + t;
+#pragma unroll
+            for (int e = 0; e < 3; e++) kx[t][e] = kz[t][e] = 0.0f;
+            kg[t][0] = kg[t][1] = 0.0f;
+            if (row_in[r]) {
+                if (L2 >= 0) {
+                    const size_t c = rowbase[r] + sk * (size_t)L2;
+                    kx[t][0] = cf[CXA * N + c];
+                    kx[t][1] = cf[CXV * N + c];
+                    kx[t][2] = cf[CXW * N + c];
+                    kg[t][0] = cf[CIG * N + c];
+                    kg[t][1] = cf[CIGZ * N + c];
+                }
+                if (L1 >= 0 && L1 <= nz - 1) {
+                    const size_t c = rowbase[r] + sk * (size_t)L1;
+                    kz[t][0] = cf[CZA * N + c];
+                    kz[t][1] = cf[CZU * N + c];
+                    kz[t][2] = cf[CZV * N + c];
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+#pragma unroll
+            for (int e = 0; e < 3; e++) ky[r][e] = 0.0f;
+            if (row_in[r] && L2 >= 0) {
+                const size_t c = rowbase[r] + sk * (size_t)L2;
+                ky[r][0] = cf[CYA * N + c];
+                ky[r][1] = cf[CYU * N + c];
+                ky[r][2] = cf[CYW * N + c];
             }
         }
         __syncthreads();
 
         // ---------------- (B) unlimited faces
-        // z face on top of level L1 for the owned columns (mpdata_fluxes :211-246)
+        // z face on top of level L1 for the owned columns
         if (L1 >= 0 && L1 <= nz - 1) {
+            const bool kz_ok = L1 <= nz - 2;
 #pragma unroll
             for (int t = 0; t < 3; t++) {
                 const int row = ly + 4 * (rb + t);
-                const int gj = j0 + row;
                 float wv = 0.0f, fv = 0.0f;
-                if (lane_out && gj >= 0 && gj < ny && L1 <= nz - 2) {
-                    const size_t c = (size_t)gi + sk * ((size_t)L1 + (size_t)nz * gj);
+                if (kz_ok && lane_out && row_in[rb + t]) {
                     const float(*qa)[CX] = sq1[L1 & 3];
                     const float(*qb)[CX] = sq1[m & 3];
                     const float l = qa[row][lx], r = qb[row][lx];
-                    const float Wc = a.Wd[c];
-                    const float rG = rcp(a.G[c + sk] + a.G[c]);
-                    const float aw = fabsf(Wc);
-                    float val = aw * (1.0f - 2.0f * aw * rG) * ((r - l) * rcp(r + l + eps));
-                    float cross = 0.0f;
-                    if (gi >= 1 && gi <= nx - 2) {  // WxU (:227-233)
-                        const float sE = qa[row][lx + 1] + qb[row][lx + 1];
-                        const float sW = qa[row][lx - 1] + qb[row][lx - 1];
-                        const float ev = 0.25f * (a.U[c + 1] + a.U[c + 1 + sk] + a.U[c] + a.U[c + sk]);
-                        cross += ev * ((sE - sW) * rcp(sE + sW + eps));
-                    }
-                    if (gj >= 1 && gj <= ny - 2) {  // WxV (:238-245)
-                        const float sN = qa[row + 1][lx] + qb[row + 1][lx];
-                        const float sS = qa[row - 1][lx] + qb[row - 1][lx];
-                        const float ev = 0.25f * (a.V[c] + a.V[c + sk] + a.V[c + sj] + a.V[c + sk + sj]);
-                        cross += ev * ((sN - sS) * rcp(sN + sS + eps));
-                    }
-                    val -= 0.5f * Wc * rG * cross;
-                    wv = val * 0.5f * a.dz[c];  // :380
+                    wv = kz[t][0] * face_ratio(l, r) -
+                         kz[t][1] * cross_ratio(qb[row][lx + 1], qa[row][lx - 1], qa[row][lx + 1], qb[row][lx - 1]) -
+                         kz[t][2] * cross_ratio(qb[row + 1][lx], qa[row - 1][lx], qa[row + 1][lx], qb[row - 1][lx]);
                     fv = dflux(l, r, wv);
                 }
                 wu2[t] = wu1[t];
@@ -283,40 +397,26 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
             const float(*q1u)[CX] = sq1[(L2 + 1) & 3];
             const float(*q1d)[CX] = sq1[(L2 + 3) & 3];
             const bool kin = L2 >= 1 && L2 <= nz - 2;
-            // x faces (left of column gi) on the owned rows, lanes 1..63 (:131-165)
+            // x faces (left of column gi) on the owned rows, lanes 1..63
 #pragma unroll
             for (int t = 0; t < 3; t++) {
                 const int row = ly + 4 * (rb + t);
-                const int gj = j0 + row;
                 float uv = 0.0f, fv = 0.0f;
-                if (lx >= 1 && gi >= 1 && gi <= nx - 1 && gj >= 0 && gj < ny) {
-                    const size_t c = (size_t)gi + sk * ((size_t)L2 + (size_t)nz * gj);
+                if (lx >= 1 && gi >= 1 && row_in[rb + t]) {
                     const float r = q1m[row][lx], l = q1m[row][lx - 1];
-                    const float Uc = a.U[c];
-                    const float rG = rcp(a.G[c] + a.G[c - 1]);
-                    const float au = fabsf(Uc);
-                    float val = au * (1.0f - 2.0f * au * rG) * ((r - l) * rcp(r + l + eps));
-                    float cross = 0.0f;
-                    if (gj >= 1 && gj <= ny - 2) {  // UxV (:145-152)
-                        const float sN = q1m[row + 1][lx] + q1m[row + 1][lx - 1];
-                        const float sS = q1m[row - 1][lx] + q1m[row - 1][lx - 1];
-                        const float ev = 0.25f * (a.V[c] + a.V[c + sj] + a.V[c - 1] + a.V[c - 1 + sj]);
-                        cross += ev * ((sN - sS) * rcp(sN + sS + eps));
-                    }
-                    if (kin) {  // UxW (:157-164)
-                        const float sU = q1u[row][lx] + q1u[row][lx - 1];
-                        const float sD = q1d[row][lx] + q1d[row][lx - 1];
-                        const float ev = 0.25f * (a.Wd[c] + a.Wd[c - sk] + a.Wd[c - 1] + a.Wd[c - 1 - sk]);
-                        cross += ev * ((sU - sD) * rcp(sU + sD + eps));
-                    }
-                    val -= 0.5f * Uc * rG * cross;
-                    uv = val * 0.5f;  // :378
+                    float val = kx[t][0] * face_ratio(l, r);
+                    val -= kx[t][1] * cross_ratio(q1m[row + 1][lx], q1m[row - 1][lx], q1m[row + 1][lx - 1],
+                                                  q1m[row - 1][lx - 1]);
+                    if (kin)
+                        val -= kx[t][2] * cross_ratio(q1u[row][lx], q1d[row][lx], q1u[row][lx - 1],
+                                                      q1d[row][lx - 1]);
+                    uv = val;
                     fv = dflux(l, r, uv);
                 }
                 sxu[row][lx] = uv;
                 sxf[row][lx] = fv;
             }
-            // y faces (south of row gj) on rows 1..15, lanes 2..61 (:169-205)
+            // y faces (south of row gj) on rows 1..15, lanes 2..61
 #pragma unroll
             for (int r4 = 0; r4 < 4; r4++) {
                 const int row = ly + 4 * r4;
@@ -324,27 +424,14 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
                 const int gj = j0 + row;
                 float vv = 0.0f, fv = 0.0f;
                 if (lane_out && gj >= 1 && gj <= ny - 1) {
-                    const size_t c = (size_t)gi + sk * ((size_t)L2 + (size_t)nz * gj);
                     const float r = q1m[row][lx], l = q1m[row - 1][lx];
-                    const float Vc = a.V[c];
-                    const float rG = rcp(a.G[c] + a.G[c - sj]);
-                    const float av = fabsf(Vc);
-                    float val = av * (1.0f - 2.0f * av * rG) * ((r - l) * rcp(r + l + eps));
-                    float cross = 0.0f;
-                    if (gi >= 1 && gi <= nx - 2) {  // VxU (:186-192)
-                        const float sE = q1m[row][lx + 1] + q1m[row - 1][lx + 1];
-                        const float sW = q1m[row][lx - 1] + q1m[row - 1][lx - 1];
-                        const float ev = 0.25f * (a.U[c + 1] + a.U[c + 1 - sj] + a.U[c] + a.U[c - sj]);
-                        cross += ev * ((sE - sW) * rcp(sE + sW + eps));
-                    }
-                    if (kin) {  // VxW (:197-204)
-                        const float sU = q1u[row][lx] + q1u[row - 1][lx];
-                        const float sD = q1d[row][lx] + q1d[row - 1][lx];
-                        const float ev = 0.25f * (a.Wd[c] + a.Wd[c - sk] + a.Wd[c - sj] + a.Wd[c - sk - sj]);
-                        cross += ev * ((sU - sD) * rcp(sU + sD + eps));
-                    }
-                    val -= 0.5f * Vc * rG * cross;
-                    vv = val * 0.5f;  // :379
+                    float val = ky[r4][0] * face_ratio(l, r);
+                    val -= ky[r4][1] * cross_ratio(q1m[row - 1][lx + 1], q1m[row][lx - 1], q1m[row][lx + 1],
+                                                   q1m[row - 1][lx - 1]);
+                    if (kin)
+                        val -= ky[r4][2] * cross_ratio(q1u[row - 1][lx], q1d[row][lx], q1u[row][lx],
+                                                       q1d[row - 1][lx]);
+                    vv = val;
                     fv = dflux(l, r, vv);
                 }
                 syu[row][lx] = vv;
@@ -500,14 +587,14 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
             for (int t = 0; t < 3; t++) {
                 const int row = ly + 4 * (rb + t);
                 const int gj = j0 + row;
-                if (lane_out && gj >= 0 && gj < ny) {
-                    const size_t c = (size_t)gi + sk * ((size_t)L2 + (size_t)nz * gj);
+                if (lane_out && row_in[rb + t]) {
+                    const size_t c = rowbase[rb + t] + sk * (size_t)L2;
                     const float qc1 = sq1[L2 & 3][row][lx];
                     float q = qc1;
                     if (!(gi == 0 || gi == nx - 1 || gj == 0 || gj == ny - 1)) {
                         const float H = (sxf[row][lx + 1] - sxf[row][lx]) + (syf[row + 1][lx] - syf[row][lx]);
-                        q = q - H * a.invG[c];
-                        q = q - (Fzc[t] - Fzp[t]) * a.invGz[c];
+                        q = q - H * kg[t][0];
+                        q = q - (Fzc[t] - Fzp[t]) * kg[t][1];
                     }
                     st_q(p.out, c, p.kindo, p.inv_scaleo, q);
                 }
@@ -537,7 +624,7 @@ struct FastState {
     std::vector<void *> alt;          // second home buffer per species
     int cur = 0;                      // which home holds the current state
     float *s1 = nullptr, *s2 = nullptr;  // FP32 scratch, nspec * N each
-    float *invG = nullptr, *invGz = nullptr;
+    float *coef = nullptr;            // NCOEF planes of N
     SpecPass *d_tab = nullptr;        // [2 parities][max passes][nspec]
     int tab_order = -1;
     size_t tab_cap = 0;
@@ -621,8 +708,7 @@ FastState *fast_create(Dims d)
     FastState *st = new FastState();
     st->d = d;
     st->N = (size_t)d.nx * d.ny * d.nz;
-    if (hipMalloc(&st->invG, st->N * sizeof(float)) != hipSuccess ||
-        hipMalloc(&st->invGz, st->N * sizeof(float)) != hipSuccess) {
+    if (hipMalloc(&st->coef, (size_t)NCOEF * st->N * sizeof(float)) != hipSuccess) {
         fast_destroy(st);
         return nullptr;
     }
@@ -633,8 +719,7 @@ void fast_destroy(FastState *st)
 {
     if (!st) return;
     free_species_buffers(st);
-    (void)hipFree(st->invG);
-    (void)hipFree(st->invGz);
+    (void)hipFree(st->coef);
     (void)hipFree(st->d_tab);
     delete st;
 }
@@ -680,26 +765,22 @@ int fast_advect(FastState *st, hipStream_t s, Dims d, const StepFields &sf, cons
     const int P = order;
     const double n = (double)st->N;
     const SpecPass *tab = st->d_tab + (size_t)st->cur * P * S;
-    timed(K_CONVERT, 4.0 * 4 * n, [&] {
-        hipLaunchKernelGGL(k_inv, dim3(grid_for(st->N)), dim3(256), 0, s, st->N, sf.G, sf.Gz,
-                           st->invG, st->invGz);
+    const dim3 cgrid((d.nx + 63) / 64, d.nz, (d.ny + 3) / 4);
+    // species-invariant coefficients: reads U, V, Wd, G, Gz, dz; writes NCOEF planes
+    timed(K_COEF, (6.0 + NCOEF) * 4 * n, [&] {
+        hipLaunchKernelGGL(k_coef, cgrid, dim3(64, 4), 0, s, d, sf.U, sf.V, sf.Wd, sf.G, sf.Gz,
+                           f.dz, st->coef);
     });
     // pass 1: donor cell for every species
     timed(K_FUSED_FIRST, (5.0 * 4 + S * 2.0 * 4) * n, [&] {
         dim3 grid((d.nx + FBX - 1) / FBX, d.nz, (d.ny + FBY - 1) / FBY);
-        hipLaunchKernelGGL(k_first, grid, dim3(FBX, FBY), 0, s, d, sf.U, sf.V, sf.W, st->invG,
-                           st->invGz, tab, S);
+        hipLaunchKernelGGL(k_first, grid, dim3(FBX, FBY), 0, s, d, sf.U, sf.V, sf.W, sf.G, sf.Gz,
+                           tab, S);
     });
     // corrective passes
     CorrArgs a;
     a.d = d;
-    a.U = sf.U;
-    a.V = sf.V;
-    a.Wd = sf.Wd;
-    a.G = sf.G;
-    a.dz = f.dz;
-    a.invG = st->invG;
-    a.invGz = st->invGz;
+    a.coef = st->coef;
     a.nspec = S;
     a.tiles_x = (d.nx + CTI - 1) / CTI;
     const int tiles_y = (d.ny + CTJ - 1) / CTJ;
@@ -707,8 +788,8 @@ int fast_advect(FastState *st, hipStream_t s, Dims d, const StepFields &sf, cons
     const int nb = (a.nwork + 7) / 8 * 8;
     for (int pass = 2; pass <= P; pass++) {
         a.sp = tab + (size_t)(pass - 1) * S;
-        // species read (q0, q1 at pass 2) and written once, invariants once
-        const double bytes = (pass == 2 ? 3.0 : 2.0) * 4 * n * S + 6.0 * 4 * n;
+        // species read (q0 and q1 at pass 2) and written once; coefficients once
+        const double bytes = (pass == 2 ? 3.0 : 2.0) * 4 * n * S + (double)NCOEF * 4 * n;
         timed(K_FUSED_CORR, bytes, [&] {
             if (fct)
                 hipLaunchKernelGGL(k_corr<true>, dim3(nb), dim3(CNT), 0, s, a);

@@ -22,9 +22,14 @@ def build() -> None:
 class Oracle:
     """FP32 (dtype=np.float32) or FP64 (np.float64) restatement."""
 
-    def __init__(self, dtype=np.float32):
+    def __init__(self, dtype=np.float32, fma: bool = False):
+        """fma=True loads the FP32 build with FMA contraction (the rounding
+        of the reference's own -O2 -mfma build) instead of the strict one."""
         self.dtype = np.dtype(dtype)
         name = "liboracle_f32.so" if self.dtype == np.float32 else "liboracle_f64.so"
+        if fma:
+            assert self.dtype == np.float32
+            name = "liboracle_f32_fma.so"
         path = os.path.join(HERE, name)
         if not os.path.exists(path):
             build()

@@ -24,3 +24,10 @@ def oracle64():
     from oracle.oracle import Oracle
     import numpy as np
     return Oracle(np.float64)
+
+
+@pytest.fixture(scope="session")
+def oracle32_fma():
+    from oracle.oracle import Oracle
+    import numpy as np
+    return Oracle(np.float32, fma=True)
