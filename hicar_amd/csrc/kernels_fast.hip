@@ -19,9 +19,14 @@
 // covers (tile, species) pairs; blocks are mapped so that the species of one
 // tile run on the same XCD and share its L2.
 //
-// Arithmetic is FP32 with reciprocals (v_rcp_f32) in place of divisions and
-// contraction allowed: results agree with the reference order to rounding,
-// not bit for bit (the exact kernels keep the bit pattern).
+// Every expression keeps the reference's evaluation order, with IEEE FP32
+// division and no contraction (this unit is compiled with
+// -ffp-contract=off), so for FP32 storage the result is bit-identical to the
+// exact kernels and to a strict IEEE FP32 build of the reference; the speed
+// comes from fusion, batching and the per-step invariants, not from cheaper
+// arithmetic.  The MPDATA step on rough fields is sensitive to rounding
+// order at the 1e-5..1e-3 level (a reordered build of the reference moves
+// theta by ~6e-4 in 5 steps), so no reordering is done.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
@@ -67,11 +72,6 @@ __device__ __forceinline__ void st_q(void *p, size_t i, int kind, float inv_scal
         static_cast<uint16_t *>(p)[i] = to16(v, kind, inv_scale);
 }
 
-__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-
-// donor-cell flux ((U+|U|) l + (U-|U|) r)/2 = U * (upwind value)
-__device__ __forceinline__ float dflux(float l, float r, float U) { return U * (U > 0.0f ? l : r); }
-
 __device__ __forceinline__ float max3(float a, float b, float c) { return fmaxf(fmaxf(a, b), c); }
 __device__ __forceinline__ float min3(float a, float b, float c) { return fminf(fminf(a, b), c); }
 
@@ -85,22 +85,22 @@ struct SpecPass {
 };
 
 // ---------------------------------------------------------------------------
-// species-invariant face coefficients, once per step (mpdata_fluxes
-// :139-151,163,180-203,221-244 with the damping of :378-380 folded in):
-//   u2 = cxa*A - cxv*A_y - cxw*A_z        (x face left of the cell)
-//   v2 = cya*A - cyu*A_x - cyw*A_z        (y face south of the cell)
-//   w2 = cza*A - czu*A_x - czv*A_y        (z face on top of the cell)
-// with A the face ratio (r-l)/(r+l+1e-10) and A_* the four-cell cross
-// ratios of the species.  Coefficients of terms the reference skips (its
-// row/level guards, missing faces) are 0.  Planes 9/10 hold 1/(jaco*rho)
-// and 1/(dz*jaco*rho) for the flux divergence.
+// species-invariant parts of the pseudo-velocities, once per step, each
+// evaluated exactly as mpdata_fluxes evaluates it (adv_mpdata.f90:139,151,
+// 163,180,192,203,221,233,244):
+//   CUx = |U|*(1-|U|/(0.5*(G_r+G_l)))   first factor of the face value
+//   CVx = (0.5*U)*edge_v                 prefix of each cross term
+//   SGx = G_r+G_l                        its divisor
+// for the x face left of a cell (x), the y face south of it (y) and the z
+// face on top of it (z).  A cross-term prefix is 0 where the reference skips
+// the term (its row/level guards); the corrective kernel tests the same
+// guards and never uses those entries.
 // ---------------------------------------------------------------------------
-enum { CXA = 0, CXV, CXW, CYA, CYU, CYW, CZA, CZU, CZV, CIG, CIGZ, NCOEF };
+enum { CUX = 0, CXV, CXW, SGX, CUY, CYU, CYW, SGY, CUZ, CZU, CZV, SGZ, NCOEF };
 
 __global__ void __launch_bounds__(256)
     k_coef(Dims d, const float *__restrict__ U, const float *__restrict__ V,
-           const float *__restrict__ Wd, const float *__restrict__ G, const float *__restrict__ Gz,
-           const float *__restrict__ dz, float *__restrict__ coef)
+           const float *__restrict__ Wd, const float *__restrict__ G, float *__restrict__ coef)
 {
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int j = blockIdx.z * 4 + threadIdx.y;
@@ -114,46 +114,44 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int e = 0; e < NCOEF; e++) v[e] = 0.0f;
     if (i >= 1) {
-        const float Uc = U[c], sG = G[c] + G[c - 1], au = fabsf(Uc);
-        v[CXA] = 0.5f * au * (1.0f - 2.0f * au / sG);
+        const float Uc = U[c], au = fabsf(Uc);
+        v[CUX] = au * (1.0f - au / (0.5f * (G[c] + G[c - 1])));
+        v[SGX] = G[c] + G[c - 1];
         if (j >= 1 && j <= ny - 2)
-            v[CXV] = 0.25f * Uc * (0.25f * (V[c] + V[c + sj] + V[c - 1] + V[c - 1 + sj])) / sG;
+            v[CXV] = 0.5f * Uc * ((1.0f / 4.0f) * (V[c] + V[c + sj] + V[c - 1] + V[c - 1 + sj]));
         if (k >= 1 && k <= nz - 2)
-            v[CXW] = 0.25f * Uc * (0.25f * (Wd[c] + Wd[c - sk] + Wd[c - 1] + Wd[c - 1 - sk])) / sG;
+            v[CXW] = 0.5f * Uc * ((1.0f / 4.0f) * (Wd[c] + Wd[c - sk] + Wd[c - 1] + Wd[c - 1 - sk]));
     }
     if (j >= 1) {
-        const float Vc = V[c], sG = G[c] + G[c - sj], av = fabsf(Vc);
-        v[CYA] = 0.5f * av * (1.0f - 2.0f * av / sG);
+        const float Vc = V[c], av = fabsf(Vc);
+        v[CUY] = av * (1.0f - av / (0.5f * (G[c] + G[c - sj])));
+        v[SGY] = G[c] + G[c - sj];
         if (i >= 1 && i <= nx - 2)
-            v[CYU] = 0.25f * Vc * (0.25f * (U[c + 1] + U[c + 1 - sj] + U[c] + U[c - sj])) / sG;
+            v[CYU] = 0.5f * Vc * ((1.0f / 4.0f) * (U[c + 1] + U[c + 1 - sj] + U[c] + U[c - sj]));
         if (k >= 1 && k <= nz - 2)
-            v[CYW] = 0.25f * Vc * (0.25f * (Wd[c] + Wd[c - sk] + Wd[c - sj] + Wd[c - sk - sj])) / sG;
+            v[CYW] = 0.5f * Vc * ((1.0f / 4.0f) * (Wd[c] + Wd[c - sk] + Wd[c - sj] + Wd[c - sk - sj]));
     }
     if (k <= nz - 2) {
-        const float Wc = Wd[c], sG = G[c + sk] + G[c], aw = fabsf(Wc), h = 0.5f * dz[c];
-        v[CZA] = h * aw * (1.0f - 2.0f * aw / sG);
+        const float Wc = Wd[c], aw = fabsf(Wc);
+        v[CUZ] = aw * (1.0f - aw / (0.5f * (G[c + sk] + G[c])));
+        v[SGZ] = G[c + sk] + G[c];
         if (i >= 1 && i <= nx - 2)
-            v[CZU] = h * 0.5f * Wc * (0.25f * (U[c + 1] + U[c + 1 + sk] + U[c] + U[c + sk])) / sG;
+            v[CZU] = 0.5f * Wc * ((1.0f / 4.0f) * (U[c + 1] + U[c + 1 + sk] + U[c] + U[c + sk]));
         if (j >= 1 && j <= ny - 2)
-            v[CZV] = h * 0.5f * Wc * (0.25f * (V[c] + V[c + sk] + V[c + sj] + V[c + sk + sj])) / sG;
+            v[CZV] = 0.5f * Wc * ((1.0f / 4.0f) * (V[c] + V[c + sk] + V[c + sj] + V[c + sk + sj]));
     }
-    v[CIG] = 1.0f / G[c];
-    v[CIGZ] = 1.0f / Gz[c];
 #pragma unroll
     for (int e = 0; e < NCOEF; e++) coef[(size_t)e * N + c] = v[e];
 }
 
 // ---------------------------------------------------------------------------
 // donor-cell pass for every species (upwind_advection, adv_mpdata.f90:43-103)
-// in the reference's operation order with IEEE division: its result is
-// bit-identical to the exact kernel's, so the first-pass field and the FCT
-// bounds built from it match the reference exactly.
 // ---------------------------------------------------------------------------
 constexpr int FBX = 64, FBY = 4;
 
-__device__ __forceinline__ float xflux1(float l, float r, float U)
+// flux1 (adv_mpdata.f90:39)
+__device__ __forceinline__ float flux1(float l, float r, float U)
 {
-#pragma clang fp contract(off)
     return ((U + fabsf(U)) * l + (U - fabsf(U)) * r) / 2.0f;
 }
 
@@ -162,7 +160,6 @@ __global__ void __launch_bounds__(FBX *FBY)
             const float *__restrict__ W, const float *__restrict__ G,
             const float *__restrict__ Gz, const SpecPass *__restrict__ sp, int nspec)
 {
-#pragma clang fp contract(off)
     const int i = blockIdx.x * FBX + threadIdx.x;
     const int j = blockIdx.z * FBY + threadIdx.y;
     const int k = blockIdx.y;
@@ -192,19 +189,19 @@ __global__ void __launch_bounds__(FBX *FBY)
         const float qe = ld_q(p.q0, c + 1, p.kind0, p.scale0);
         const float qs = ld_q(p.q0, c - sj, p.kind0, p.scale0);
         const float qn = ld_q(p.q0, c + sj, p.kind0, p.scale0);
-        const float fr = xflux1(qc, qe, Ur), fl = xflux1(qw, qc, Ul);
-        const float fn = xflux1(qc, qn, Vn), fs = xflux1(qs, qc, Vs);
+        const float fr = flux1(qc, qe, Ur), fl = flux1(qw, qc, Ul);
+        const float fn = flux1(qc, qn, Vn), fs = flux1(qs, qc, Vs);
         float q = qc - ((fr - fl) + (fn - fs)) / g;  // :86-87
         if (k == 0) {
             const float qu = ld_q(p.q0, c + sk, p.kind0, p.scale0);
-            q = q - xflux1(qc, qu, Wt) / gz;  // :93-94
+            q = q - flux1(qc, qu, Wt) / gz;  // :93-94
         } else if (k == d.nz - 1) {
             const float qd = ld_q(p.q0, c - sk, p.kind0, p.scale0);
-            q = q - (qc * Wt - xflux1(qd, qc, Wb)) / gz;  // top flux not upwinded (:96-97)
+            q = q - (qc * Wt - flux1(qd, qc, Wb)) / gz;  // top flux not upwinded (:96-97)
         } else {
             const float qu = ld_q(p.q0, c + sk, p.kind0, p.scale0);
             const float qd = ld_q(p.q0, c - sk, p.kind0, p.scale0);
-            q = q - (xflux1(qc, qu, Wt) - xflux1(qd, qc, Wb)) / gz;  // :90-91
+            q = q - (flux1(qc, qu, Wt) - flux1(qd, qc, Wb)) / gz;  // :90-91
         }
         st_q(p.out, c, p.kindo, p.inv_scaleo, q);
     }
@@ -223,17 +220,10 @@ constexpr int CNT = 256;         // 4 wavefronts; wave ly owns rows ly, ly+4, ly
 struct CorrArgs {
     Dims d;
     const float *coef;  // NCOEF planes of N
+    const float *G, *Gz, *dz;
     const SpecPass *sp;
     int nspec, tiles_x, nwork;
 };
-
-// four-cell cross ratio ((a-b)+(c-d)) / ((a+b)+(c+d)+1e-10)
-__device__ __forceinline__ float cross_ratio(float a, float b, float c, float d)
-{
-    return ((a - b) + (c - d)) * rcp(((a + b) + (c + d)) + 1e-10f);
-}
-
-__device__ __forceinline__ float face_ratio(float l, float r) { return (r - l) * rcp(r + l + 1e-10f); }
 
 template <bool FCT>
 __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
@@ -255,12 +245,12 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
     const int j0 = ty * CTJ - CH;  // global row of staged row 0
     const SpecPass p = a.sp[s];
     const bool same = p.q1 == nullptr;
-    const float eps15 = 1e-15f;
+    const float eps = 1e-10f, eps15 = 1e-15f;
     const float *__restrict__ cf = a.coef;
 
     __shared__ float sq1[4][CY][CX];  // q1 at levels m-3..m (ring by level & 3)
     __shared__ float sq0[3][CY][CX];  // q0 at levels m-2..m (ring by level % 3)
-    __shared__ float sxu[CY][CX], sxf[CY][CX];  // x face left of a column: U2, donor flux
+    __shared__ float sxu[CY][CX], sxf[CY][CX];  // x face left of a column: U2, flux1
     __shared__ float syu[CY][CX], syf[CY][CX];  // y face south of a row
     __shared__ float sbxi[CY][CX], sbxo[CY][CX], sbyi[CY][CX], sbyo[CY][CX];  // FCT betas
 
@@ -268,9 +258,8 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
     const bool lane_out = lx >= CH && lx < CX - CH && lane_in;
     const int rb = (ly < 2) ? 1 : 0;  // own interior rows: ly + 4*(rb + t), t = 0..2
 
-    // own rows / staged rows in the domain
     bool row_in[4];
-    size_t rowbase[4];  // gi + sj*gj (level offset added per use)
+    size_t rowbase[4];  // gi + sj*gj (the level offset is added per use)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         const int gj = j0 + ly + 4 * r;
@@ -280,9 +269,9 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
 
     // per owned column (3 interior rows): z pipeline state
     float z0a[3], z0b[3], z0c[3];          // q0 at levels m-2, m-1, m
-    float wu2[3], wu1[3], fz2[3], fz1[3];  // unlimited w / z donor flux at faces m-2, m-1
+    float wu2[3], wu1[3], fz2[3], fz1[3];  // unlimited w / its flux1 on faces m-2, m-1
     float bzi[3], bzo[3];                  // betas of the cell below the current one
-    float Fzp[3];                          // limited z flux on the face below the output level
+    float Fzp[3];                          // flux on the face below the output level
 #pragma unroll
     for (int t = 0; t < 3; t++) {
         z0a[t] = z0b[t] = z0c[t] = 0.0f;
@@ -304,8 +293,8 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
 
     for (int m = 0; m <= nz + 1; m++) {
         const int L1 = m - 1, L2 = m - 2;
-        // ---------------- (A) stage level m, prefetch level m+1 and this
-        //                  iteration's coefficients
+        // ---------------- (A) stage level m, prefetch level m+1 and the
+        //                  coefficients of this iteration
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int row = ly + 4 * r;
@@ -330,61 +319,77 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
                 }
             }
         }
-        // coefficients: x faces and divergence at L2 (own rows), y faces at
-        // L2 (rows 1..15), z faces at L1 (own rows)
-        float kx[3][3], kz[3][3], ky[4][3], kg[3][2];
+        // x faces and divergence at L2 (own rows), y faces at L2 (rows
+        // 1..15), z faces at L1 (own rows)
+        float kx[3][4], kz[3][5], ky[4][4], kg[3][2];
 #pragma unroll
         for (int t = 0; t < 3; t++) {
             const int r = rb + t;
 #pragma unroll
-            for (int e = 0; e < 3; e++) kx[t][e] = kz[t][e] = 0.0f;
-            kg[t][0] = kg[t][1] = 0.0f;
+            for (int e = 0; e < 4; e++) kx[t][e] = 0.0f;
+#pragma unroll
+            for (int e = 0; e < 5; e++) kz[t][e] = 0.0f;
+            kg[t][0] = kg[t][1] = 1.0f;
             if (row_in[r]) {
                 if (L2 >= 0) {
                     const size_t c = rowbase[r] + sk * (size_t)L2;
-                    kx[t][0] = cf[CXA * N + c];
+                    kx[t][0] = cf[CUX * N + c];
                     kx[t][1] = cf[CXV * N + c];
                     kx[t][2] = cf[CXW * N + c];
-                    kg[t][0] = cf[CIG * N + c];
-                    kg[t][1] = cf[CIGZ * N + c];
+                    kx[t][3] = cf[SGX * N + c];
+                    kg[t][0] = a.G[c];
+                    kg[t][1] = a.Gz[c];
                 }
-                if (L1 >= 0 && L1 <= nz - 1) {
+                if (L1 >= 0 && L1 <= nz - 2) {
                     const size_t c = rowbase[r] + sk * (size_t)L1;
-                    kz[t][0] = cf[CZA * N + c];
+                    kz[t][0] = cf[CUZ * N + c];
                     kz[t][1] = cf[CZU * N + c];
                     kz[t][2] = cf[CZV * N + c];
+                    kz[t][3] = cf[SGZ * N + c];
+                    kz[t][4] = a.dz[c];
                 }
             }
         }
 #pragma unroll
         for (int r = 0; r < 4; r++) {
 #pragma unroll
-            for (int e = 0; e < 3; e++) ky[r][e] = 0.0f;
+            for (int e = 0; e < 4; e++) ky[r][e] = 0.0f;
             if (row_in[r] && L2 >= 0) {
                 const size_t c = rowbase[r] + sk * (size_t)L2;
-                ky[r][0] = cf[CYA * N + c];
+                ky[r][0] = cf[CUY * N + c];
                 ky[r][1] = cf[CYU * N + c];
                 ky[r][2] = cf[CYW * N + c];
+                ky[r][3] = cf[SGY * N + c];
             }
         }
         __syncthreads();
 
-        // ---------------- (B) unlimited faces
-        // z face on top of level L1 for the owned columns
+        // ---------------- (B) unlimited pseudo-velocities (mpdata_fluxes)
+        // z face on top of level L1 for the owned columns (:211-246, :380)
         if (L1 >= 0 && L1 <= nz - 1) {
             const bool kz_ok = L1 <= nz - 2;
 #pragma unroll
             for (int t = 0; t < 3; t++) {
                 const int row = ly + 4 * (rb + t);
+                const int gj = j0 + row;
                 float wv = 0.0f, fv = 0.0f;
                 if (kz_ok && lane_out && row_in[rb + t]) {
-                    const float(*qa)[CX] = sq1[L1 & 3];
-                    const float(*qb)[CX] = sq1[m & 3];
+                    const float(*qa)[CX] = sq1[L1 & 3];  // level k
+                    const float(*qb)[CX] = sq1[m & 3];   // level k+1
                     const float l = qa[row][lx], r = qb[row][lx];
-                    wv = kz[t][0] * face_ratio(l, r) -
-                         kz[t][1] * cross_ratio(qb[row][lx + 1], qa[row][lx - 1], qa[row][lx + 1], qb[row][lx - 1]) -
-                         kz[t][2] * cross_ratio(qb[row + 1][lx], qa[row - 1][lx], qa[row + 1][lx], qb[row - 1][lx]);
-                    fv = dflux(l, r, wv);
+                    float val = kz[t][0] * (r - l) / (r + l + eps);
+                    if (gi >= 1 && gi <= nx - 2) {  // WxU (:227-233)
+                        const float eq = (qb[row][lx + 1] - qa[row][lx - 1] + qa[row][lx + 1] - qb[row][lx - 1]) /
+                                         (qa[row][lx + 1] + qb[row][lx + 1] + qa[row][lx - 1] + qb[row][lx - 1] + eps);
+                        val = val - kz[t][1] * eq / kz[t][3];
+                    }
+                    if (gj >= 1 && gj <= ny - 2) {  // WxV (:238-245)
+                        const float eq = (qb[row + 1][lx] - qa[row - 1][lx] + qa[row + 1][lx] - qb[row - 1][lx]) /
+                                         (qa[row + 1][lx] + qb[row - 1][lx] + qb[row + 1][lx] + qa[row - 1][lx] + eps);
+                        val = val - kz[t][2] * eq / kz[t][3];
+                    }
+                    wv = val * 0.5f * kz[t][4];
+                    fv = flux1(l, r, wv);
                 }
                 wu2[t] = wu1[t];
                 fz2[t] = fz1[t];
@@ -397,26 +402,32 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
             const float(*q1u)[CX] = sq1[(L2 + 1) & 3];
             const float(*q1d)[CX] = sq1[(L2 + 3) & 3];
             const bool kin = L2 >= 1 && L2 <= nz - 2;
-            // x faces (left of column gi) on the owned rows, lanes 1..63
+            // x faces (left of column gi) on the owned rows, lanes 1..63 (:131-165, :378)
 #pragma unroll
             for (int t = 0; t < 3; t++) {
                 const int row = ly + 4 * (rb + t);
+                const int gj = j0 + row;
                 float uv = 0.0f, fv = 0.0f;
                 if (lx >= 1 && gi >= 1 && row_in[rb + t]) {
                     const float r = q1m[row][lx], l = q1m[row][lx - 1];
-                    float val = kx[t][0] * face_ratio(l, r);
-                    val -= kx[t][1] * cross_ratio(q1m[row + 1][lx], q1m[row - 1][lx], q1m[row + 1][lx - 1],
-                                                  q1m[row - 1][lx - 1]);
-                    if (kin)
-                        val -= kx[t][2] * cross_ratio(q1u[row][lx], q1d[row][lx], q1u[row][lx - 1],
-                                                      q1d[row][lx - 1]);
-                    uv = val;
-                    fv = dflux(l, r, uv);
+                    float val = kx[t][0] * (r - l) / (r + l + eps);
+                    if (gj >= 1 && gj <= ny - 2) {  // UxV (:145-152)
+                        const float eq = (q1m[row + 1][lx] - q1m[row - 1][lx] + q1m[row + 1][lx - 1] - q1m[row - 1][lx - 1]) /
+                                         (q1m[row + 1][lx] + q1m[row - 1][lx] + q1m[row + 1][lx - 1] + q1m[row - 1][lx - 1] + eps);
+                        val = val - kx[t][1] * eq / kx[t][3];
+                    }
+                    if (kin) {  // UxW (:157-164)
+                        const float eq = (q1u[row][lx] - q1d[row][lx] + q1u[row][lx - 1] - q1d[row][lx - 1]) /
+                                         (q1u[row][lx] + q1d[row][lx] + q1u[row][lx - 1] + q1d[row][lx - 1] + eps);
+                        val = val - kx[t][2] * eq / kx[t][3];
+                    }
+                    uv = val * 0.5f;
+                    fv = flux1(l, r, uv);
                 }
                 sxu[row][lx] = uv;
                 sxf[row][lx] = fv;
             }
-            // y faces (south of row gj) on rows 1..15, lanes 2..61
+            // y faces (south of row gj) on rows 1..15, lanes 2..61 (:169-205, :379)
 #pragma unroll
             for (int r4 = 0; r4 < 4; r4++) {
                 const int row = ly + 4 * r4;
@@ -425,14 +436,19 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
                 float vv = 0.0f, fv = 0.0f;
                 if (lane_out && gj >= 1 && gj <= ny - 1) {
                     const float r = q1m[row][lx], l = q1m[row - 1][lx];
-                    float val = ky[r4][0] * face_ratio(l, r);
-                    val -= ky[r4][1] * cross_ratio(q1m[row - 1][lx + 1], q1m[row][lx - 1], q1m[row][lx + 1],
-                                                   q1m[row - 1][lx - 1]);
-                    if (kin)
-                        val -= ky[r4][2] * cross_ratio(q1u[row - 1][lx], q1d[row][lx], q1u[row][lx],
-                                                       q1d[row - 1][lx]);
-                    vv = val;
-                    fv = dflux(l, r, vv);
+                    float val = ky[r4][0] * (r - l) / (r + l + eps);
+                    if (gi >= 1 && gi <= nx - 2) {  // VxU (:186-192)
+                        const float eq = (q1m[row - 1][lx + 1] - q1m[row][lx - 1] + q1m[row][lx + 1] - q1m[row - 1][lx - 1]) /
+                                         (q1m[row][lx + 1] + q1m[row - 1][lx + 1] + q1m[row][lx - 1] + q1m[row - 1][lx - 1] + eps);
+                        val = val - ky[r4][1] * eq / ky[r4][3];
+                    }
+                    if (kin) {  // VxW (:197-204)
+                        const float eq = (q1u[row - 1][lx] - q1d[row][lx] + q1u[row][lx] - q1d[row - 1][lx]) /
+                                         (q1u[row - 1][lx] + q1d[row][lx] + q1u[row][lx] + q1d[row - 1][lx] + eps);
+                        val = val - ky[r4][2] * eq / ky[r4][3];
+                    }
+                    vv = val * 0.5f;
+                    fv = flux1(l, r, vv);
                 }
                 syu[row][lx] = vv;
                 syf[row][lx] = fv;
@@ -441,7 +457,7 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
         __syncthreads();
 
         // ---------------- (C) FCT: per-cell betas (adv_mpdata_FCT_core.f90:47-116)
-        float Fzc[3] = {0.0f, 0.0f, 0.0f};  // limited z flux on top of level L2
+        float Fzc[3] = {0.0f, 0.0f, 0.0f};  // flux on top of level L2
         if (FCT) {
             if (L2 >= 0) {
                 const float(*q1m)[CX] = sq1[L2 & 3];
@@ -468,8 +484,8 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
                             fin = fmaxf(0.0f, fl) - fminf(0.0f, fr);
                             fout = fmaxf(0.0f, fr) - fminf(0.0f, fl);
                         }
-                        sbxi[row][lx] = (qmax - qc1) * rcp(fin + eps15);
-                        sbxo[row][lx] = (qc1 - qmin) * rcp(fout + eps15);
+                        sbxi[row][lx] = (qmax - qc1) / (fin + eps15);
+                        sbxo[row][lx] = (qc1 - qmin) / (fout + eps15);
                     }
                 }
                 // y cells on rows 1..14, lanes 2..61
@@ -496,13 +512,13 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
                             fin = fmaxf(0.0f, fs) - fminf(0.0f, fn);
                             fout = fmaxf(0.0f, fn) - fminf(0.0f, fs);
                         }
-                        sbyi[row][lx] = (qmax - qc1) * rcp(fin + eps15);
-                        sbyo[row][lx] = (qc1 - qmin) * rcp(fout + eps15);
+                        sbyi[row][lx] = (qmax - qc1) / (fin + eps15);
+                        sbyo[row][lx] = (qc1 - qmin) / (fout + eps15);
                     }
                 }
             }
-            // z: betas of cell L1 of each owned column, then the limited flux
-            // on the face between L2 and L1 (flux_is_w end rules)
+            // z: betas of cell L1 of each owned column, then the limited
+            // velocity and flux on the face between L2 and L1 (flux_is_w ends)
             if (L1 >= 0 && L1 <= nz - 1) {
 #pragma unroll
                 for (int t = 0; t < 3; t++) {
@@ -520,7 +536,7 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
                         qmax = max3(qd1, qc1, z0a[t]);
                         qmin = min3(qd1, qc1, z0a[t]);
                         fin = fmaxf(0.0f, fz2[t]) - fminf(0.0f, fz2[t]);
-                        fout = fin;
+                        fout = fmaxf(0.0f, fz2[t]) - fminf(0.0f, fz2[t]);
                     } else {
                         const float qd1 = sq1[L2 & 3][row][lx], qu1 = sq1[m & 3][row][lx];
                         qmax = fmaxf(max3(qd1, qc1, qu1), max3(z0a[t], z0b[t], z0c[t]));
@@ -528,16 +544,15 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
                         fin = fmaxf(0.0f, fz2[t]) - fminf(0.0f, fz1[t]);
                         fout = fmaxf(0.0f, fz1[t]) - fminf(0.0f, fz2[t]);
                     }
-                    const float bi = (qmax - qc1) * rcp(fin + eps15);
-                    const float bo = (qc1 - qmin) * rcp(fout + eps15);
+                    const float bi = (qmax - qc1) / (fin + eps15);
+                    const float bo = (qc1 - qmin) / (fout + eps15);
                     if (L2 >= 0 && L2 <= nz - 2) {
-                        const float u = wu2[t];
-                        float fac = 0.0f;
-                        if (u > 0.0f)
-                            fac = fminf(1.0f, fminf(bi, bzo[t]));
-                        else if (u < 0.0f)
-                            fac = fminf(1.0f, fminf(bzi[t], bo));
-                        Fzc[t] = fac * fz2[t];
+                        float wl = wu2[t];
+                        if (wl > 0.0f)
+                            wl = fminf(fminf(1.0f, bi), bzo[t]) * wl;
+                        else if (wl < 0.0f)
+                            wl = fminf(fminf(1.0f, bzi[t]), bo) * wl;
+                        Fzc[t] = flux1(sq1[L2 & 3][row][lx], qc1, wl);
                     }
                     bzi[t] = bi;
                     bzo[t] = bo;
@@ -549,19 +564,19 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
         }
         __syncthreads();
 
-        // ---------------- (D) limited x/y donor fluxes (in place of the unlimited ones)
+        // ---------------- (D) limited velocities and their fluxes (in place)
         if (FCT && L2 >= 0) {
+            const float(*q1m)[CX] = sq1[L2 & 3];
 #pragma unroll
             for (int t = 0; t < 3; t++) {
                 const int row = ly + 4 * (rb + t);
                 if (lx >= 2 && lx <= CX - 2) {
-                    const float u = sxu[row][lx];
-                    float fac = 0.0f;
+                    float u = sxu[row][lx];
                     if (u > 0.0f)
-                        fac = fminf(1.0f, fminf(sbxi[row][lx], sbxo[row][lx - 1]));
+                        u = fminf(fminf(1.0f, sbxi[row][lx]), sbxo[row][lx - 1]) * u;
                     else if (u < 0.0f)
-                        fac = fminf(1.0f, fminf(sbxi[row][lx - 1], sbxo[row][lx]));
-                    sxf[row][lx] = fac * sxf[row][lx];
+                        u = fminf(fminf(1.0f, sbxi[row][lx - 1]), sbxo[row][lx]) * u;
+                    sxf[row][lx] = flux1(q1m[row][lx - 1], q1m[row][lx], u);
                 }
             }
 #pragma unroll
@@ -569,19 +584,18 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
                 const int row = ly + 4 * r4;
                 if (row < 2 || row > CY - 2) continue;
                 if (lane_out) {
-                    const float u = syu[row][lx];
-                    float fac = 0.0f;
-                    if (u > 0.0f)
-                        fac = fminf(1.0f, fminf(sbyi[row][lx], sbyo[row - 1][lx]));
-                    else if (u < 0.0f)
-                        fac = fminf(1.0f, fminf(sbyi[row - 1][lx], sbyo[row][lx]));
-                    syf[row][lx] = fac * syf[row][lx];
+                    float v = syu[row][lx];
+                    if (v > 0.0f)
+                        v = fminf(fminf(1.0f, sbyi[row][lx]), sbyo[row - 1][lx]) * v;
+                    else if (v < 0.0f)
+                        v = fminf(fminf(1.0f, sbyi[row - 1][lx]), sbyo[row][lx]) * v;
+                    syf[row][lx] = flux1(q1m[row - 1][lx], q1m[row][lx], v);
                 }
             }
         }
         __syncthreads();
 
-        // ---------------- (E) flux divergence on level L2 (upwind_advection with U2)
+        // ---------------- (E) flux divergence on level L2 (upwind_advection :86-97)
         if (L2 >= 0) {
 #pragma unroll
             for (int t = 0; t < 3; t++) {
@@ -592,9 +606,8 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
                     const float qc1 = sq1[L2 & 3][row][lx];
                     float q = qc1;
                     if (!(gi == 0 || gi == nx - 1 || gj == 0 || gj == ny - 1)) {
-                        const float H = (sxf[row][lx + 1] - sxf[row][lx]) + (syf[row + 1][lx] - syf[row][lx]);
-                        q = q - H * kg[t][0];
-                        q = q - (Fzc[t] - Fzp[t]) * kg[t][1];
+                        q = q - ((sxf[row][lx + 1] - sxf[row][lx]) + (syf[row + 1][lx] - syf[row][lx])) / kg[t][0];
+                        q = q - (Fzc[t] - Fzp[t]) / kg[t][1];
                     }
                     st_q(p.out, c, p.kindo, p.inv_scaleo, q);
                 }
@@ -767,9 +780,8 @@ int fast_advect(FastState *st, hipStream_t s, Dims d, const StepFields &sf, cons
     const SpecPass *tab = st->d_tab + (size_t)st->cur * P * S;
     const dim3 cgrid((d.nx + 63) / 64, d.nz, (d.ny + 3) / 4);
     // species-invariant coefficients: reads U, V, Wd, G, Gz, dz; writes NCOEF planes
-    timed(K_COEF, (6.0 + NCOEF) * 4 * n, [&] {
-        hipLaunchKernelGGL(k_coef, cgrid, dim3(64, 4), 0, s, d, sf.U, sf.V, sf.Wd, sf.G, sf.Gz,
-                           f.dz, st->coef);
+    timed(K_COEF, (4.0 + NCOEF) * 4 * n, [&] {
+        hipLaunchKernelGGL(k_coef, cgrid, dim3(64, 4), 0, s, d, sf.U, sf.V, sf.Wd, sf.G, st->coef);
     });
     // pass 1: donor cell for every species
     timed(K_FUSED_FIRST, (5.0 * 4 + S * 2.0 * 4) * n, [&] {
@@ -781,6 +793,9 @@ int fast_advect(FastState *st, hipStream_t s, Dims d, const StepFields &sf, cons
     CorrArgs a;
     a.d = d;
     a.coef = st->coef;
+    a.G = sf.G;
+    a.Gz = sf.Gz;
+    a.dz = f.dz;
     a.nspec = S;
     a.tiles_x = (d.nx + CTI - 1) / CTI;
     const int tiles_y = (d.ny + CTJ - 1) / CTJ;
@@ -789,7 +804,7 @@ int fast_advect(FastState *st, hipStream_t s, Dims d, const StepFields &sf, cons
     for (int pass = 2; pass <= P; pass++) {
         a.sp = tab + (size_t)(pass - 1) * S;
         // species read (q0 and q1 at pass 2) and written once; coefficients once
-        const double bytes = (pass == 2 ? 3.0 : 2.0) * 4 * n * S + (double)NCOEF * 4 * n;
+        const double bytes = (pass == 2 ? 3.0 : 2.0) * 4 * n * S + (NCOEF + 3.0) * 4 * n;
         timed(K_FUSED_CORR, bytes, [&] {
             if (fct)
                 hipLaunchKernelGGL(k_corr<true>, dim3(nb), dim3(CNT), 0, s, a);
