@@ -12,8 +12,8 @@
 //              the 1-D FCT limiter in x, y and z (flux_limiter, :254-350 and
 //              adv_mpdata_FCT_core.f90) and the donor-cell flux divergence
 //              with the limited velocities, fused in one kernel.
-// k_corr marches upward through the levels of a 60 x 12 column tile (halo 2,
-// 64 x 16 columns staged in LDS); x/y neighbours come from LDS planes, the z
+// k_corr marches upward through the levels of a 60 x 16 column tile (halo 2,
+// 64 x 20 columns staged in LDS); x/y neighbours come from LDS planes, the z
 // direction (including the column-wise FCT) stays in registers of the thread
 // that owns the column.  Species are independent inside advect, so a launch
 // covers (tile, species) pairs; blocks are mapped so that the species of one
@@ -30,6 +30,7 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -210,12 +211,15 @@ __global__ void __launch_bounds__(FBX *FBY)
 // ---------------------------------------------------------------------------
 // fused corrective pass
 // ---------------------------------------------------------------------------
-constexpr int CX = 64;           // staged tile width (one wavefront per row)
-constexpr int CY = 16;           // staged tile rows
-constexpr int CH = 2;            // halo
-constexpr int CTI = CX - 2 * CH; // 60 output columns
-constexpr int CTJ = CY - 2 * CH; // 12 output rows
-constexpr int CNT = 256;         // 4 wavefronts; wave ly owns rows ly, ly+4, ly+8, ly+12
+constexpr int CX = 64;            // staged tile width (one wavefront per row)
+constexpr int CY = 20;            // staged tile rows
+constexpr int CH = 2;             // halo
+constexpr int CTI = CX - 2 * CH;  // 60 output columns
+constexpr int CTJ = CY - 2 * CH;  // 16 output rows
+constexpr int NW = 8;             // wavefronts; wave ly stages rows ly, ly+8, ly+16
+constexpr int CNT = 64 * NW;
+constexpr int SR = 3;             // staged-row slots per wave (the third only for ly < 4)
+constexpr int OWN = 2;            // output rows per wave: ly + 8*(rb + t), t = 0, 1
 
 struct CorrArgs {
     Dims d;
@@ -225,8 +229,11 @@ struct CorrArgs {
     int nspec, tiles_x, nwork;
 };
 
-template <bool FCT>
-__global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
+// WPE: waves per SIMD the register allocation targets (2 blocks of 8 waves
+// per CU need 4); PF: load this iteration's coefficients before the first
+// barrier (latency hidden by the barrier) or at their use.
+template <bool FCT, int WPE, bool PF>
+__global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) k_corr(CorrArgs a)
 {
     // XCD-aware mapping: consecutive work items (the species of one tile)
     // land on the same XCD (blocks are dealt round-robin over 8 XCDs) and
@@ -238,42 +245,50 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
     const int tile = w / a.nspec;
     const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
     const int nx = a.d.nx, ny = a.d.ny, nz = a.d.nz;
+    // 32-bit element offsets (the host checks nx*ny*nz < 2^32) so that the
+    // loads use a scalar base and one VGPR offset
+    const unsigned sk = (unsigned)nx, sj = (unsigned)nx * (unsigned)nz;
     const size_t N = (size_t)nx * ny * nz;
-    const size_t sk = (size_t)nx, sj = (size_t)nx * nz;
     const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
     const int gi = tx * CTI - CH + lx;
     const int j0 = ty * CTJ - CH;  // global row of staged row 0
     const SpecPass p = a.sp[s];
     const bool same = p.q1 == nullptr;
     const float eps = 1e-10f, eps15 = 1e-15f;
-    const float *__restrict__ cf = a.coef;
+    const float *__restrict__ cux = a.coef + CUX * N, *__restrict__ cxv = a.coef + CXV * N;
+    const float *__restrict__ cxw = a.coef + CXW * N, *__restrict__ sgx = a.coef + SGX * N;
+    const float *__restrict__ cuy = a.coef + CUY * N, *__restrict__ cyu = a.coef + CYU * N;
+    const float *__restrict__ cyw = a.coef + CYW * N, *__restrict__ sgy = a.coef + SGY * N;
+    const float *__restrict__ cuz = a.coef + CUZ * N, *__restrict__ czu = a.coef + CZU * N;
+    const float *__restrict__ czv = a.coef + CZV * N, *__restrict__ sgz = a.coef + SGZ * N;
 
     __shared__ float sq1[4][CY][CX];  // q1 at levels m-3..m (ring by level & 3)
     __shared__ float sq0[3][CY][CX];  // q0 at levels m-2..m (ring by level % 3)
-    __shared__ float sxu[CY][CX], sxf[CY][CX];  // x face left of a column: U2, flux1
+    __shared__ float sxu[CY][CX], sxf[CY][CX];  // x face left of a column: U2, flux1(U2)
     __shared__ float syu[CY][CX], syf[CY][CX];  // y face south of a row
     __shared__ float sbxi[CY][CX], sbxo[CY][CX], sbyi[CY][CX], sbyo[CY][CX];  // FCT betas
 
     const bool lane_in = gi >= 0 && gi < nx;
     const bool lane_out = lx >= CH && lx < CX - CH && lane_in;
-    const int rb = (ly < 2) ? 1 : 0;  // own interior rows: ly + 4*(rb + t), t = 0..2
+    const int rb = (ly < 2) ? 1 : 0;
+    const int nsr = (ly + 8 * (SR - 1) < CY) ? SR : SR - 1;  // staged rows of this wave
 
-    bool row_in[4];
-    size_t rowbase[4];  // gi + sj*gj (the level offset is added per use)
+    bool row_in[SR];
+    unsigned rowbase[SR];  // gi + sj*gj (the level offset is added per use)
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const int gj = j0 + ly + 4 * r;
-        row_in[r] = lane_in && gj >= 0 && gj < ny;
-        rowbase[r] = row_in[r] ? (size_t)gi + sj * (size_t)gj : 0;
+    for (int r = 0; r < SR; r++) {
+        const int gj = j0 + ly + 8 * r;
+        row_in[r] = r < nsr && lane_in && gj >= 0 && gj < ny;
+        rowbase[r] = row_in[r] ? (unsigned)gi + sj * (unsigned)gj : 0u;
     }
 
-    // per owned column (3 interior rows): z pipeline state
-    float z0a[3], z0b[3], z0c[3];          // q0 at levels m-2, m-1, m
-    float wu2[3], wu1[3], fz2[3], fz1[3];  // unlimited w / its flux1 on faces m-2, m-1
-    float bzi[3], bzo[3];                  // betas of the cell below the current one
-    float Fzp[3];                          // flux on the face below the output level
+    // per owned column: z pipeline state
+    float z0a[OWN], z0b[OWN], z0c[OWN];          // q0 at levels m-2, m-1, m
+    float wu2[OWN], wu1[OWN], fz2[OWN], fz1[OWN];  // unlimited w / its flux1 on faces m-2, m-1
+    float bzi[OWN], bzo[OWN];                    // betas of the cell below the current one
+    float Fzp[OWN];                              // flux on the face below the output level
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
+    for (int t = 0; t < OWN; t++) {
         z0a[t] = z0b[t] = z0c[t] = 0.0f;
         wu2[t] = wu1[t] = fz2[t] = fz1[t] = 0.0f;
         bzi[t] = bzo[t] = 1.0f;
@@ -281,9 +296,9 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
     }
 
     // register prefetch of the next staged level
-    float n0[4], n1[4];
+    float n0[SR], n1[SR];
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
+    for (int r = 0; r < SR; r++) {
         n0[r] = n1[r] = 0.0f;
         if (row_in[r]) {
             n0[r] = ld_q(p.q0, rowbase[r], p.kind0, p.scale0);
@@ -296,13 +311,13 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
         // ---------------- (A) stage level m, prefetch level m+1 and the
         //                  coefficients of this iteration
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int row = ly + 4 * r;
-            if (m < nz) {
+        for (int r = 0; r < SR; r++) {
+            const int row = ly + 8 * r;
+            if (m < nz && r < nsr) {
                 sq0[m % 3][row][lx] = n0[r];
                 sq1[m & 3][row][lx] = n1[r];
             }
-            if (r >= rb && r < rb + 3) {  // own column: q0 at levels m-2, m-1, m
+            if (r >= rb && r < rb + OWN) {  // own column: q0 at levels m-2, m-1, m
                 const int t = r - rb;
                 z0a[t] = z0b[t];
                 z0b[t] = z0c[t];
@@ -311,66 +326,70 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
         }
         if (m + 1 < nz) {
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
+            for (int r = 0; r < SR; r++) {
                 if (row_in[r]) {
-                    const size_t c = rowbase[r] + sk * (size_t)(m + 1);
+                    const unsigned c = rowbase[r] + sk * (unsigned)(m + 1);
                     n0[r] = ld_q(p.q0, c, p.kind0, p.scale0);
                     n1[r] = same ? n0[r] : p.q1[c];
                 }
             }
         }
-        // x faces and divergence at L2 (own rows), y faces at L2 (rows
-        // 1..15), z faces at L1 (own rows)
-        float kx[3][4], kz[3][5], ky[4][4], kg[3][2];
+        // x faces and divergence at L2 (own rows), y faces at L2 (staged
+        // rows), z faces at L1 (own rows)
+        float kx[OWN][4], kz[OWN][5], ky[SR][4], kg[OWN][2];
+        auto load_coef = [&]() {
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
-            const int r = rb + t;
+            for (int t = 0; t < OWN; t++) {
+                const int r = rb + t;
 #pragma unroll
-            for (int e = 0; e < 4; e++) kx[t][e] = 0.0f;
+                for (int e = 0; e < 4; e++) kx[t][e] = 0.0f;
 #pragma unroll
-            for (int e = 0; e < 5; e++) kz[t][e] = 0.0f;
-            kg[t][0] = kg[t][1] = 1.0f;
-            if (row_in[r]) {
-                if (L2 >= 0) {
-                    const size_t c = rowbase[r] + sk * (size_t)L2;
-                    kx[t][0] = cf[CUX * N + c];
-                    kx[t][1] = cf[CXV * N + c];
-                    kx[t][2] = cf[CXW * N + c];
-                    kx[t][3] = cf[SGX * N + c];
-                    kg[t][0] = a.G[c];
-                    kg[t][1] = a.Gz[c];
-                }
-                if (L1 >= 0 && L1 <= nz - 2) {
-                    const size_t c = rowbase[r] + sk * (size_t)L1;
-                    kz[t][0] = cf[CUZ * N + c];
-                    kz[t][1] = cf[CZU * N + c];
-                    kz[t][2] = cf[CZV * N + c];
-                    kz[t][3] = cf[SGZ * N + c];
-                    kz[t][4] = a.dz[c];
+                for (int e = 0; e < 5; e++) kz[t][e] = 0.0f;
+                kg[t][0] = kg[t][1] = 1.0f;
+                if (row_in[r]) {
+                    if (L2 >= 0) {
+                        const unsigned c = rowbase[r] + sk * (unsigned)L2;
+                        kx[t][0] = cux[c];
+                        kx[t][1] = cxv[c];
+                        kx[t][2] = cxw[c];
+                        kx[t][3] = sgx[c];
+                        kg[t][0] = a.G[c];
+                        kg[t][1] = a.Gz[c];
+                    }
+                    if (L1 >= 0 && L1 <= nz - 2) {
+                        const unsigned c = rowbase[r] + sk * (unsigned)L1;
+                        kz[t][0] = cuz[c];
+                        kz[t][1] = czu[c];
+                        kz[t][2] = czv[c];
+                        kz[t][3] = sgz[c];
+                        kz[t][4] = a.dz[c];
+                    }
                 }
             }
-        }
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
+            for (int r = 0; r < SR; r++) {
 #pragma unroll
-            for (int e = 0; e < 4; e++) ky[r][e] = 0.0f;
-            if (row_in[r] && L2 >= 0) {
-                const size_t c = rowbase[r] + sk * (size_t)L2;
-                ky[r][0] = cf[CUY * N + c];
-                ky[r][1] = cf[CYU * N + c];
-                ky[r][2] = cf[CYW * N + c];
-                ky[r][3] = cf[SGY * N + c];
+                for (int e = 0; e < 4; e++) ky[r][e] = 0.0f;
+                if (row_in[r] && L2 >= 0) {
+                    const unsigned c = rowbase[r] + sk * (unsigned)L2;
+                    ky[r][0] = cuy[c];
+                    ky[r][1] = cyu[c];
+                    ky[r][2] = cyw[c];
+                    ky[r][3] = sgy[c];
+                }
             }
-        }
+        };
+        if (PF) load_coef();
         __syncthreads();
+        if (!PF) load_coef();
 
         // ---------------- (B) unlimited pseudo-velocities (mpdata_fluxes)
         // z face on top of level L1 for the owned columns (:211-246, :380)
         if (L1 >= 0 && L1 <= nz - 1) {
             const bool kz_ok = L1 <= nz - 2;
 #pragma unroll
-            for (int t = 0; t < 3; t++) {
-                const int row = ly + 4 * (rb + t);
+            for (int t = 0; t < OWN; t++) {
+                const int row = ly + 8 * (rb + t);
                 const int gj = j0 + row;
                 float wv = 0.0f, fv = 0.0f;
                 if (kz_ok && lane_out && row_in[rb + t]) {
@@ -404,8 +423,8 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
             const bool kin = L2 >= 1 && L2 <= nz - 2;
             // x faces (left of column gi) on the owned rows, lanes 1..63 (:131-165, :378)
 #pragma unroll
-            for (int t = 0; t < 3; t++) {
-                const int row = ly + 4 * (rb + t);
+            for (int t = 0; t < OWN; t++) {
+                const int row = ly + 8 * (rb + t);
                 const int gj = j0 + row;
                 float uv = 0.0f, fv = 0.0f;
                 if (lx >= 1 && gi >= 1 && row_in[rb + t]) {
@@ -427,11 +446,11 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
                 sxu[row][lx] = uv;
                 sxf[row][lx] = fv;
             }
-            // y faces (south of row gj) on rows 1..15, lanes 2..61 (:169-205, :379)
+            // y faces (south of row gj) on the staged rows 1..19, lanes 2..61 (:169-205, :379)
 #pragma unroll
-            for (int r4 = 0; r4 < 4; r4++) {
-                const int row = ly + 4 * r4;
-                if (row < 1) continue;
+            for (int r4 = 0; r4 < SR; r4++) {
+                const int row = ly + 8 * r4;
+                if (r4 >= nsr || row < 1) continue;
                 const int gj = j0 + row;
                 float vv = 0.0f, fv = 0.0f;
                 if (lane_out && gj >= 1 && gj <= ny - 1) {
@@ -457,15 +476,17 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
         __syncthreads();
 
         // ---------------- (C) FCT: per-cell betas (adv_mpdata_FCT_core.f90:47-116)
-        float Fzc[3] = {0.0f, 0.0f, 0.0f};  // flux on top of level L2
+        float Fzc[OWN];  // flux on top of level L2
+#pragma unroll
+        for (int t = 0; t < OWN; t++) Fzc[t] = 0.0f;
         if (FCT) {
             if (L2 >= 0) {
                 const float(*q1m)[CX] = sq1[L2 & 3];
                 const float(*q0m)[CX] = sq0[L2 % 3];
                 // x cells of the owned rows, lanes 1..62
 #pragma unroll
-                for (int t = 0; t < 3; t++) {
-                    const int row = ly + 4 * (rb + t);
+                for (int t = 0; t < OWN; t++) {
+                    const int row = ly + 8 * (rb + t);
                     if (lx >= 1 && lx <= CX - 2 && lane_in) {
                         float qmax, qmin, fin = 0.0f, fout = 0.0f;
                         const float qc1 = q1m[row][lx];
@@ -488,11 +509,11 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
                         sbxo[row][lx] = (qc1 - qmin) / (fout + eps15);
                     }
                 }
-                // y cells on rows 1..14, lanes 2..61
+                // y cells on the staged rows 1..18, lanes 2..61
 #pragma unroll
-                for (int r4 = 0; r4 < 4; r4++) {
-                    const int row = ly + 4 * r4;
-                    if (row < 1 || row > CY - 2) continue;
+                for (int r4 = 0; r4 < SR; r4++) {
+                    const int row = ly + 8 * r4;
+                    if (r4 >= nsr || row < 1 || row > CY - 2) continue;
                     const int gj = j0 + row;
                     if (lane_out && gj >= 0 && gj < ny) {
                         float qmax, qmin, fin = 0.0f, fout = 0.0f;
@@ -521,8 +542,8 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
             // velocity and flux on the face between L2 and L1 (flux_is_w ends)
             if (L1 >= 0 && L1 <= nz - 1) {
 #pragma unroll
-                for (int t = 0; t < 3; t++) {
-                    const int row = ly + 4 * (rb + t);
+                for (int t = 0; t < OWN; t++) {
+                    const int row = ly + 8 * (rb + t);
                     const float qc1 = sq1[L1 & 3][row][lx];
                     float qmax, qmin, fin, fout;
                     if (L1 == 0) {
@@ -560,53 +581,57 @@ __global__ void __launch_bounds__(CNT) k_corr(CorrArgs a)
             }
         } else {
 #pragma unroll
-            for (int t = 0; t < 3; t++) Fzc[t] = (L2 >= 0 && L2 <= nz - 2) ? fz2[t] : 0.0f;
+            for (int t = 0; t < OWN; t++) Fzc[t] = (L2 >= 0 && L2 <= nz - 2) ? fz2[t] : 0.0f;
         }
         __syncthreads();
 
-        // ---------------- (D) limited velocities and their fluxes (in place)
-        if (FCT && L2 >= 0) {
+        // ---------------- (D) flux divergence on level L2 (upwind_advection
+        // :86-97) with the limited velocities of the cell's four side faces;
+        // a face shared by two cells is limited identically by both
+        if (L2 >= 0) {
             const float(*q1m)[CX] = sq1[L2 & 3];
 #pragma unroll
-            for (int t = 0; t < 3; t++) {
-                const int row = ly + 4 * (rb + t);
-                if (lx >= 2 && lx <= CX - 2) {
-                    float u = sxu[row][lx];
-                    if (u > 0.0f)
-                        u = fminf(fminf(1.0f, sbxi[row][lx]), sbxo[row][lx - 1]) * u;
-                    else if (u < 0.0f)
-                        u = fminf(fminf(1.0f, sbxi[row][lx - 1]), sbxo[row][lx]) * u;
-                    sxf[row][lx] = flux1(q1m[row][lx - 1], q1m[row][lx], u);
-                }
-            }
-#pragma unroll
-            for (int r4 = 0; r4 < 4; r4++) {
-                const int row = ly + 4 * r4;
-                if (row < 2 || row > CY - 2) continue;
-                if (lane_out) {
-                    float v = syu[row][lx];
-                    if (v > 0.0f)
-                        v = fminf(fminf(1.0f, sbyi[row][lx]), sbyo[row - 1][lx]) * v;
-                    else if (v < 0.0f)
-                        v = fminf(fminf(1.0f, sbyi[row - 1][lx]), sbyo[row][lx]) * v;
-                    syf[row][lx] = flux1(q1m[row - 1][lx], q1m[row][lx], v);
-                }
-            }
-        }
-        __syncthreads();
-
-        // ---------------- (E) flux divergence on level L2 (upwind_advection :86-97)
-        if (L2 >= 0) {
-#pragma unroll
-            for (int t = 0; t < 3; t++) {
-                const int row = ly + 4 * (rb + t);
+            for (int t = 0; t < OWN; t++) {
+                const int row = ly + 8 * (rb + t);
                 const int gj = j0 + row;
                 if (lane_out && row_in[rb + t]) {
-                    const size_t c = rowbase[rb + t] + sk * (size_t)L2;
-                    const float qc1 = sq1[L2 & 3][row][lx];
+                    const unsigned c = rowbase[rb + t] + sk * (unsigned)L2;
+                    const float qc1 = q1m[row][lx];
                     float q = qc1;
                     if (!(gi == 0 || gi == nx - 1 || gj == 0 || gj == ny - 1)) {
-                        q = q - ((sxf[row][lx + 1] - sxf[row][lx]) + (syf[row + 1][lx] - syf[row][lx])) / kg[t][0];
+                        float fl, fr, fs, fn;
+                        if (FCT) {
+                            float u = sxu[row][lx];  // face between lx-1 and lx
+                            if (u > 0.0f)
+                                u = fminf(fminf(1.0f, sbxi[row][lx]), sbxo[row][lx - 1]) * u;
+                            else if (u < 0.0f)
+                                u = fminf(fminf(1.0f, sbxi[row][lx - 1]), sbxo[row][lx]) * u;
+                            fl = flux1(q1m[row][lx - 1], qc1, u);
+                            u = sxu[row][lx + 1];  // face between lx and lx+1
+                            if (u > 0.0f)
+                                u = fminf(fminf(1.0f, sbxi[row][lx + 1]), sbxo[row][lx]) * u;
+                            else if (u < 0.0f)
+                                u = fminf(fminf(1.0f, sbxi[row][lx]), sbxo[row][lx + 1]) * u;
+                            fr = flux1(qc1, q1m[row][lx + 1], u);
+                            float v = syu[row][lx];  // face between row-1 and row
+                            if (v > 0.0f)
+                                v = fminf(fminf(1.0f, sbyi[row][lx]), sbyo[row - 1][lx]) * v;
+                            else if (v < 0.0f)
+                                v = fminf(fminf(1.0f, sbyi[row - 1][lx]), sbyo[row][lx]) * v;
+                            fs = flux1(q1m[row - 1][lx], qc1, v);
+                            v = syu[row + 1][lx];  // face between row and row+1
+                            if (v > 0.0f)
+                                v = fminf(fminf(1.0f, sbyi[row + 1][lx]), sbyo[row][lx]) * v;
+                            else if (v < 0.0f)
+                                v = fminf(fminf(1.0f, sbyi[row][lx]), sbyo[row + 1][lx]) * v;
+                            fn = flux1(qc1, q1m[row + 1][lx], v);
+                        } else {
+                            fl = sxf[row][lx];
+                            fr = sxf[row][lx + 1];
+                            fs = syf[row][lx];
+                            fn = syf[row + 1][lx];
+                        }
+                        q = q - ((fr - fl) + (fn - fs)) / kg[t][0];
                         q = q - (Fzc[t] - Fzp[t]) / kg[t][1];
                     }
                     st_q(p.out, c, p.kindo, p.inv_scaleo, q);
@@ -641,6 +666,7 @@ struct FastState {
     SpecPass *d_tab = nullptr;        // [2 parities][max passes][nspec]
     int tab_order = -1;
     size_t tab_cap = 0;
+    int corr_cfg = 1;  // corrective-kernel configuration: bit 0 WPE 4, bit 1 no prefetch
 };
 
 namespace {
@@ -720,6 +746,7 @@ FastState *fast_create(Dims d)
 {
     FastState *st = new FastState();
     st->d = d;
+    if (const char *e = getenv("ICAR_ADV_CORR_CFG")) st->corr_cfg = atoi(e) & 3;
     st->N = (size_t)d.nx * d.ny * d.nz;
     if (hipMalloc(&st->coef, (size_t)NCOEF * st->N * sizeof(float)) != hipSuccess) {
         fast_destroy(st);
@@ -798,6 +825,10 @@ int fast_advect(FastState *st, hipStream_t s, Dims d, const StepFields &sf, cons
     a.dz = f.dz;
     a.nspec = S;
     a.tiles_x = (d.nx + CTI - 1) / CTI;
+    if (st->N >= 4294967296ull) {
+        st->err = "the fused path needs nx*ny*nz < 2^32";
+        return ICAR_ADV_ERANGE;
+    }
     const int tiles_y = (d.ny + CTJ - 1) / CTJ;
     a.nwork = a.tiles_x * tiles_y * S;
     const int nb = (a.nwork + 7) / 8 * 8;
@@ -806,10 +837,16 @@ int fast_advect(FastState *st, hipStream_t s, Dims d, const StepFields &sf, cons
         // species read (q0 and q1 at pass 2) and written once; coefficients once
         const double bytes = (pass == 2 ? 3.0 : 2.0) * 4 * n * S + (NCOEF + 3.0) * 4 * n;
         timed(K_FUSED_CORR, bytes, [&] {
-            if (fct)
-                hipLaunchKernelGGL(k_corr<true>, dim3(nb), dim3(CNT), 0, s, a);
-            else
-                hipLaunchKernelGGL(k_corr<false>, dim3(nb), dim3(CNT), 0, s, a);
+            switch ((fct ? 4 : 0) + st->corr_cfg) {
+            case 0: hipLaunchKernelGGL((k_corr<false, 2, true>), dim3(nb), dim3(CNT), 0, s, a); break;
+            case 1: hipLaunchKernelGGL((k_corr<false, 4, true>), dim3(nb), dim3(CNT), 0, s, a); break;
+            case 2: hipLaunchKernelGGL((k_corr<false, 2, false>), dim3(nb), dim3(CNT), 0, s, a); break;
+            case 3: hipLaunchKernelGGL((k_corr<false, 4, false>), dim3(nb), dim3(CNT), 0, s, a); break;
+            case 4: hipLaunchKernelGGL((k_corr<true, 2, true>), dim3(nb), dim3(CNT), 0, s, a); break;
+            case 5: hipLaunchKernelGGL((k_corr<true, 4, true>), dim3(nb), dim3(CNT), 0, s, a); break;
+            case 6: hipLaunchKernelGGL((k_corr<true, 2, false>), dim3(nb), dim3(CNT), 0, s, a); break;
+            default: hipLaunchKernelGGL((k_corr<true, 4, false>), dim3(nb), dim3(CNT), 0, s, a); break;
+            }
         });
     }
     if (P <= 2) st->cur = 1 - st->cur;  // the result landed in the other home buffer
