@@ -95,7 +95,9 @@ struct SpecPass {
 // for the x face left of a cell (x), the y face south of it (y) and the z
 // face on top of it (z).  A cross-term prefix is 0 where the reference skips
 // the term (its row/level guards); the corrective kernel tests the same
-// guards and never uses those entries.
+// guards and never uses those entries.  Stored as three float4 groups per
+// cell (x, y, z faces) so that the corrective kernel loads a face's four
+// values with one 128-bit load.
 // ---------------------------------------------------------------------------
 enum { CUX = 0, CXV, CXW, SGX, CUY, CYU, CYW, SGY, CUZ, CZU, CZV, SGZ, NCOEF };
 
@@ -141,8 +143,10 @@ __global__ void __launch_bounds__(256)
         if (j >= 1 && j <= ny - 2)
             v[CZV] = 0.5f * Wc * ((1.0f / 4.0f) * (V[c] + V[c + sk] + V[c + sj] + V[c + sk + sj]));
     }
-#pragma unroll
-    for (int e = 0; e < NCOEF; e++) coef[(size_t)e * N + c] = v[e];
+    float4 *g = reinterpret_cast<float4 *>(coef);
+    g[c] = make_float4(v[CUX], v[CXV], v[CXW], v[SGX]);
+    g[N + c] = make_float4(v[CUY], v[CYU], v[CYW], v[SGY]);
+    g[2 * N + c] = make_float4(v[CUZ], v[CZU], v[CZV], v[SGZ]);
 }
 
 // ---------------------------------------------------------------------------
@@ -255,12 +259,9 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
     const SpecPass p = a.sp[s];
     const bool same = p.q1 == nullptr;
     const float eps = 1e-10f, eps15 = 1e-15f;
-    const float *__restrict__ cux = a.coef + CUX * N, *__restrict__ cxv = a.coef + CXV * N;
-    const float *__restrict__ cxw = a.coef + CXW * N, *__restrict__ sgx = a.coef + SGX * N;
-    const float *__restrict__ cuy = a.coef + CUY * N, *__restrict__ cyu = a.coef + CYU * N;
-    const float *__restrict__ cyw = a.coef + CYW * N, *__restrict__ sgy = a.coef + SGY * N;
-    const float *__restrict__ cuz = a.coef + CUZ * N, *__restrict__ czu = a.coef + CZU * N;
-    const float *__restrict__ czv = a.coef + CZV * N, *__restrict__ sgz = a.coef + SGZ * N;
+    const float4 *__restrict__ gx = reinterpret_cast<const float4 *>(a.coef);
+    const float4 *__restrict__ gy = gx + N;
+    const float4 *__restrict__ gz = gx + 2 * N;
 
     __shared__ float sq1[4][CY][CX];  // q1 at levels m-3..m (ring by level & 3)
     __shared__ float sq0[3][CY][CX];  // q0 at levels m-2..m (ring by level % 3)
@@ -349,19 +350,21 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                 if (row_in[r]) {
                     if (L2 >= 0) {
                         const unsigned c = rowbase[r] + sk * (unsigned)L2;
-                        kx[t][0] = cux[c];
-                        kx[t][1] = cxv[c];
-                        kx[t][2] = cxw[c];
-                        kx[t][3] = sgx[c];
+                        const float4 v = gx[c];
+                        kx[t][0] = v.x;
+                        kx[t][1] = v.y;
+                        kx[t][2] = v.z;
+                        kx[t][3] = v.w;
                         kg[t][0] = a.G[c];
                         kg[t][1] = a.Gz[c];
                     }
                     if (L1 >= 0 && L1 <= nz - 2) {
                         const unsigned c = rowbase[r] + sk * (unsigned)L1;
-                        kz[t][0] = cuz[c];
-                        kz[t][1] = czu[c];
-                        kz[t][2] = czv[c];
-                        kz[t][3] = sgz[c];
+                        const float4 v = gz[c];
+                        kz[t][0] = v.x;
+                        kz[t][1] = v.y;
+                        kz[t][2] = v.z;
+                        kz[t][3] = v.w;
                         kz[t][4] = a.dz[c];
                     }
                 }
@@ -372,10 +375,11 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                 for (int e = 0; e < 4; e++) ky[r][e] = 0.0f;
                 if (row_in[r] && L2 >= 0) {
                     const unsigned c = rowbase[r] + sk * (unsigned)L2;
-                    ky[r][0] = cuy[c];
-                    ky[r][1] = cyu[c];
-                    ky[r][2] = cyw[c];
-                    ky[r][3] = sgy[c];
+                    const float4 v = gy[c];
+                    ky[r][0] = v.x;
+                    ky[r][1] = v.y;
+                    ky[r][2] = v.z;
+                    ky[r][3] = v.w;
                 }
             }
         };
