@@ -1,0 +1,169 @@
+!> ----------------------------------------------------------------------------
+!!  iso_c_binding interface to libicar_adv.so, the MI355X MPDATA advection
+!!  library (C-ABI declared in include/icar_adv.h).
+!!
+!!  ICAR's advection module (src/physics/advection_driver.f90) calls
+!!  icar_adv_create from adv_init, binds domain%u/v/w/density, the jacobians
+!!  and advection_dz plus every domain%adv_vars entry, and forwards
+!!  advect(domain, options, dt) to icar_adv_advect for adv=2.
+!! ----------------------------------------------------------------------------
+module icar_adv_c
+    use iso_c_binding
+    implicit none
+    private
+
+    public :: icar_adv_grid, icar_adv_opts
+    public :: icar_adv_create, icar_adv_destroy, icar_adv_bind_field, icar_adv_bind_species
+    public :: icar_adv_upload, icar_adv_download, icar_adv_advect, icar_adv_mass_budget
+    public :: icar_adv_max_courant, icar_adv_last_error, icar_adv_sync, icar_adv_apply_bc
+    public :: icar_adv_run, icar_adv_set_graph, icar_adv_error_message
+    public :: ICAR_ADV_FIELD_U, ICAR_ADV_FIELD_V, ICAR_ADV_FIELD_W, ICAR_ADV_FIELD_DENSITY
+    public :: ICAR_ADV_FIELD_JACOBIAN, ICAR_ADV_FIELD_JACOBIAN_U, ICAR_ADV_FIELD_JACOBIAN_V
+    public :: ICAR_ADV_FIELD_JACOBIAN_W, ICAR_ADV_FIELD_ADVECTION_DZ
+    public :: ICAR_ADV_STORE_F32, ICAR_ADV_STORE_BF16, ICAR_ADV_STORE_F16
+    public :: ICAR_ADV_MASK_ALL_FIELDS, ICAR_ADV_MASK_ALL_SPECIES
+
+    integer(c_int), parameter :: ICAR_ADV_FIELD_U = 0, ICAR_ADV_FIELD_V = 1, ICAR_ADV_FIELD_W = 2
+    integer(c_int), parameter :: ICAR_ADV_FIELD_DENSITY = 3, ICAR_ADV_FIELD_JACOBIAN = 4
+    integer(c_int), parameter :: ICAR_ADV_FIELD_JACOBIAN_U = 5, ICAR_ADV_FIELD_JACOBIAN_V = 6
+    integer(c_int), parameter :: ICAR_ADV_FIELD_JACOBIAN_W = 7, ICAR_ADV_FIELD_ADVECTION_DZ = 8
+    integer(c_int), parameter :: ICAR_ADV_STORE_F32 = 0, ICAR_ADV_STORE_BF16 = 1, ICAR_ADV_STORE_F16 = 2
+    integer(c_long_long), parameter :: ICAR_ADV_MASK_ALL_FIELDS = int(z'1FF', c_long_long)
+    ! bits 16..63 (species 0..47); written as a negative two's-complement value
+    integer(c_long_long), parameter :: ICAR_ADV_MASK_ALL_SPECIES = -65536_c_long_long
+
+    !> bounds and spacing copied by mpdata_init from domain_t (adv_mpdata.f90:422-431,486)
+    type, bind(C) :: icar_adv_grid
+        integer(c_int) :: ims, ime, kms, kme, jms, jme
+        integer(c_int) :: its, ite, jts, jte
+        real(c_float)  :: dx
+    end type
+
+    !> namelist switches: mpdata_order, MPDATA_FCT, advect_density, debug, RK3
+    !! (options_obj.f90:1568,1579-1584,1025,2198) and the kernel variant
+    type, bind(C) :: icar_adv_opts
+        integer(c_int) :: mpdata_order = 2
+        integer(c_int) :: mpdata_fct = 1
+        integer(c_int) :: advect_density = 0
+        integer(c_int) :: debug = 0
+        integer(c_int) :: rk3 = 0
+        integer(c_int) :: kernel_variant = 1
+    end type
+
+    interface
+        integer(c_int) function icar_adv_create(grid, opts, handle) bind(C, name="icar_adv_create")
+            import :: c_int, c_ptr, icar_adv_grid, icar_adv_opts
+            type(icar_adv_grid), intent(in) :: grid
+            type(icar_adv_opts), intent(in) :: opts
+            type(c_ptr), intent(out) :: handle
+        end function
+
+        integer(c_int) function icar_adv_destroy(handle) bind(C, name="icar_adv_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+        end function
+
+        integer(c_int) function icar_adv_bind_field(handle, field_id, host, lb, ub) &
+                bind(C, name="icar_adv_bind_field")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+            integer(c_int), value :: field_id
+            type(c_ptr), value :: host
+            integer(c_int), intent(in) :: lb(3), ub(3)
+        end function
+
+        integer(c_int) function icar_adv_bind_species(handle, idx, host, storage, pow2_exp) &
+                bind(C, name="icar_adv_bind_species")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+            integer(c_int), value :: idx, storage, pow2_exp
+            type(c_ptr), value :: host
+        end function
+
+        integer(c_int) function icar_adv_upload(handle, mask) bind(C, name="icar_adv_upload")
+            import :: c_int, c_ptr, c_long_long
+            type(c_ptr), value :: handle
+            integer(c_long_long), value :: mask
+        end function
+
+        integer(c_int) function icar_adv_download(handle, mask) bind(C, name="icar_adv_download")
+            import :: c_int, c_ptr, c_long_long
+            type(c_ptr), value :: handle
+            integer(c_long_long), value :: mask
+        end function
+
+        integer(c_int) function icar_adv_advect(handle, dt) bind(C, name="icar_adv_advect")
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: handle
+            real(c_float), value :: dt
+        end function
+
+        integer(c_int) function icar_adv_mass_budget(handle, per_species) &
+                bind(C, name="icar_adv_mass_budget")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: handle
+            real(c_double), intent(out) :: per_species(*)
+        end function
+
+        integer(c_int) function icar_adv_max_courant(handle, out) bind(C, name="icar_adv_max_courant")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: handle
+            real(c_double), intent(out) :: out(3)
+        end function
+
+        type(c_ptr) function icar_adv_last_error(handle) bind(C, name="icar_adv_last_error")
+            import :: c_ptr
+            type(c_ptr), value :: handle
+        end function
+
+        integer(c_int) function icar_adv_sync(handle) bind(C, name="icar_adv_sync")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+        end function
+
+        integer(c_int) function icar_adv_apply_bc(handle, bc_x, bc_y) bind(C, name="icar_adv_apply_bc")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+            integer(c_int), value :: bc_x, bc_y
+        end function
+
+        integer(c_int) function icar_adv_run(handle, dt, nsteps, bc_x, bc_y) bind(C, name="icar_adv_run")
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: handle
+            real(c_float), value :: dt
+            integer(c_int), value :: nsteps, bc_x, bc_y
+        end function
+
+        integer(c_int) function icar_adv_set_graph(handle, enable) bind(C, name="icar_adv_set_graph")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+            integer(c_int), value :: enable
+        end function
+    end interface
+
+contains
+
+    !> The library's message for the last failure on handle, as a Fortran
+    !! string (for `error stop`, as ICAR stops on errors, time_step.f90:165).
+    function icar_adv_error_message(handle) result(msg)
+        type(c_ptr), intent(in) :: handle
+        character(len=:), allocatable :: msg
+        character(kind=c_char), pointer :: chars(:)
+        type(c_ptr) :: p
+        integer :: n
+        p = icar_adv_last_error(handle)
+        if (.not. c_associated(p)) then
+            msg = ""
+            return
+        end if
+        call c_f_pointer(p, chars, [4096])
+        n = 0
+        do while (n < 4096)
+            if (chars(n + 1) == c_null_char) exit
+            n = n + 1
+        end do
+        allocate(character(len=n) :: msg)
+        msg = transfer(chars(1:n), msg)
+    end function
+
+end module icar_adv_c

@@ -33,7 +33,9 @@ EXPORTS = (
     "icar_adv_upload", "icar_adv_download", "icar_adv_advect", "icar_adv_mass_budget",
     "icar_adv_max_courant", "icar_adv_last_error", "icar_adv_sync", "icar_adv_apply_bc",
     "icar_adv_run", "icar_adv_profile", "icar_adv_kernel_count", "icar_adv_kernel_name",
-    "icar_adv_kernel_stats",
+    "icar_adv_kernel_stats", "icar_adv_set_graph", "icar_adv_ensemble_create",
+    "icar_adv_ensemble_destroy", "icar_adv_ensemble_run", "icar_adv_ensemble_sync",
+    "icar_adv_ensemble_last_error", "icar_adv_ensemble_profile", "icar_adv_ensemble_kernel_stats",
 )
 
 
@@ -89,6 +91,14 @@ def load() -> ctypes.CDLL:
         "icar_adv_kernel_count": ([], ip),
         "icar_adv_kernel_name": ([ip], ctypes.c_char_p),
         "icar_adv_kernel_stats": ([vp, ip, dp, ctypes.POINTER(ctypes.c_longlong), dp], ip),
+        "icar_adv_set_graph": ([vp, ip], ip),
+        "icar_adv_ensemble_create": ([ip, ctypes.POINTER(vp), ctypes.POINTER(vp)], ip),
+        "icar_adv_ensemble_destroy": ([vp], ip),
+        "icar_adv_ensemble_run": ([vp, fp, ip, ip, ip, ip], ip),
+        "icar_adv_ensemble_sync": ([vp], ip),
+        "icar_adv_ensemble_last_error": ([vp], ctypes.c_char_p),
+        "icar_adv_ensemble_profile": ([vp, ip], ip),
+        "icar_adv_ensemble_kernel_stats": ([vp, ip, dp, ctypes.POINTER(ctypes.c_longlong), dp], ip),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -101,4 +111,10 @@ def load() -> ctypes.CDLL:
 def check(rc: int, handle=None) -> None:
     if rc != 0:
         msg = load().icar_adv_last_error(handle)
+        raise IcarAdvError(rc, msg.decode() if msg else "")
+
+
+def check_ens(rc: int, ens=None) -> None:
+    if rc != 0:
+        msg = load().icar_adv_ensemble_last_error(ens)
         raise IcarAdvError(rc, msg.decode() if msg else "")

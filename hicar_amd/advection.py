@@ -106,6 +106,10 @@ class MpdataAdvection:
             self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
+    def set_graph(self, enable: bool) -> None:
+        """Record the run() loop in a HIP graph and replay it."""
+        self._check(self.lib.icar_adv_set_graph(self.h, int(enable)))
+
     def profile(self, enable: bool) -> None:
         self._check(self.lib.icar_adv_profile(self.h, int(enable)))
 
@@ -121,6 +125,57 @@ class MpdataAdvection:
                 out[self.lib.icar_adv_kernel_name(kid).decode()] = {
                     "total_ms": ms.value, "launches": n.value, "alg_bytes": b.value}
         return out
+
+
+class Ensemble:
+    """Independent domains (one MpdataAdvection each, kernel_variant 1)
+    advanced together: one launch per pass over every (domain, tile,
+    species), the step loop optionally recorded in a HIP graph."""
+
+    def __init__(self, members: list):
+        self.lib = _lib.load()
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        e = ctypes.c_void_p()
+        _lib.check_ens(self.lib.icar_adv_ensemble_create(len(self.members), arr, ctypes.byref(e)))
+        self.e = e
+
+    def _check(self, rc):
+        _lib.check_ens(rc, self.e)
+
+    def run(self, dts, nsteps: int, bc_x: int = 0, bc_y: int = 0, graph: bool = False) -> None:
+        dt = np.ascontiguousarray(dts, dtype=np.float32)
+        assert dt.shape == (len(self.members),)
+        self._check(self.lib.icar_adv_ensemble_run(self.e, dt.ctypes.data_as(_fp), nsteps, bc_x,
+                                                    bc_y, int(graph)))
+
+    def sync(self) -> None:
+        self._check(self.lib.icar_adv_ensemble_sync(self.e))
+
+    def profile(self, enable: bool) -> None:
+        self._check(self.lib.icar_adv_ensemble_profile(self.e, int(enable)))
+
+    def kernel_stats(self) -> dict:
+        out = {}
+        for kid in range(self.lib.icar_adv_kernel_count()):
+            ms, n, b = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double()
+            self._check(self.lib.icar_adv_ensemble_kernel_stats(self.e, kid, ctypes.byref(ms),
+                                                                ctypes.byref(n), ctypes.byref(b)))
+            if n.value:
+                out[self.lib.icar_adv_kernel_name(kid).decode()] = {
+                    "total_ms": ms.value, "launches": n.value, "alg_bytes": b.value}
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "e", None) is not None and self.e.value:
+            self.lib.icar_adv_ensemble_destroy(self.e)
+            self.e = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def adv_var_request(options: Options) -> None:

@@ -59,6 +59,12 @@ struct Handle {
     std::vector<double> h_partial;
     std::string err;
     FastState *fast = nullptr;
+    FastBatch *batch = nullptr;  // this domain as a batch of one (fused path)
+    // graph capture of icar_adv_run
+    bool use_graph = false;
+    hipGraphExec_t gexec = nullptr;
+    float g_dt = 0.0f;
+    int g_steps = -1, g_bcx = -1, g_bcy = -1, g_par0 = -1, g_par1 = -1;
     // profiling
     bool prof = false;
     std::vector<TimedLaunch> pending;
@@ -89,7 +95,8 @@ int set_err(Handle *h, int code, const char *fmt, ...)
             return set_err((h), (int)e_, "%s failed: %s", #expr, hipGetErrorString(e_));     \
     } while (0)
 
-hipEvent_t pool_event(Handle *h)
+template <class T>
+hipEvent_t pool_event(T *h)
 {
     if (!h->event_pool.empty()) {
         hipEvent_t e = h->event_pool.back();
@@ -102,8 +109,8 @@ hipEvent_t pool_event(Handle *h)
 }
 
 // Records an event pair around launch() when profiling is on.
-template <class F>
-void timed(Handle *h, int id, double bytes, F &&launch)
+template <class T, class F>
+void timed(T *h, int id, double bytes, F &&launch)
 {
     if (!h->prof) {
         launch();
@@ -120,7 +127,8 @@ void timed(Handle *h, int id, double bytes, F &&launch)
     h->pending.push_back({id, a, b, bytes});
 }
 
-void drain_profile(Handle *h)
+template <class T>
+void drain_profile(T *h)
 {
     for (auto &t : h->pending) {
         float ms = 0.0f;
@@ -187,6 +195,8 @@ void free_all(Handle *h)
     for (float *p : bufs) (void)hipFree(p);
     for (auto &s : h->sp) (void)hipFree(s.dev);
     (void)hipFree(h->d_partial);
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    fast_batch_destroy(h->batch);
     fast_destroy(h->fast);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -273,6 +283,44 @@ int advect_species_exact(Handle *h, float *q)
     return 0;
 }
 
+int ensure_batch(Handle *h)
+{
+    if (h->batch) return 0;
+    FastDomain fd{h->fast, domain_fields(h), h->sf, h->g.dx, h->o.advect_density};
+    std::string e;
+    h->batch = fast_batch_create(&fd, 1, h->o.mpdata_order, h->o.mpdata_fct, &e);
+    if (!h->batch) return set_err(h, ICAR_ADV_EINVAL, "%s", e.c_str());
+    return 0;
+}
+
+template <class T>
+TimedFn timed_fn(T *h)
+{
+    return [h](int id, double bytes, const std::function<void()> &fn) { timed(h, id, bytes, fn); };
+}
+
+// test_divergence (adv_mpdata.f90:500-502): the reference prints a warning
+int divergence_check(Handle *h)
+{
+    const int nb = mass_partials_blocks(h->d);
+    launch_divergence(h->stream, h->d, h->sf.U, h->sf.V, h->sf.W,
+                      h->dev_field[ICAR_ADV_FIELD_ADVECTION_DZ], h->d_partial);
+    HIPCHK(h, hipMemcpyAsync(h->h_partial.data(), h->d_partial, sizeof(double) * 2 * nb,
+                             hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double cnt = 0, mx = 0;
+    for (int b = 0; b < nb; b++) {
+        cnt += h->h_partial[2 * b];
+        mx = mx > h->h_partial[2 * b + 1] ? mx : h->h_partial[2 * b + 1];
+    }
+    if (cnt > 0)
+        fprintf(stderr,
+                "icar_adv: winds are not balanced on entry to advect: %.0f cells with "
+                "|div| > 1e-3 (max %g)\n",
+                cnt, mx);
+    return 0;
+}
+
 int do_advect(Handle *h, float dt)
 {
     int rc = require_bound(h);
@@ -281,39 +329,27 @@ int do_advect(Handle *h, float dt)
     h->have_dt = true;
     const DomainFields f = domain_fields(h);
     const double n = cells(h);
+    if (h->o.kernel_variant == 1 && !h->o.rk3 && h->nspecies > 0 && h->o.mpdata_order >= 1) {
+        // fused path: Courant numbers, coefficients and every pass of every
+        // species in one launch per pass
+        rc = ensure_batch(h);
+        if (rc) return rc;
+        std::string e;
+        rc = fast_batch_advect(h->batch, h->stream, &dt, timed_fn(h), &e);
+        if (rc) return set_err(h, rc, "%s", e.c_str());
+        HIPCHK(h, hipGetLastError());
+        if (h->o.debug) return divergence_check(h);
+        return 0;
+    }
     timed(h, K_COURANT, (9 + 6) * 4 * n, [&] {
         launch_courant(h->stream, h->d, h->g.dx, dt, h->o.advect_density, f, h->sf);
     });
     HIPCHK(h, hipGetLastError());
     if (h->o.debug) {
-        // test_divergence (adv_mpdata.f90:500-502): the reference prints a warning
-        const int nb = mass_partials_blocks(h->d);
-        launch_divergence(h->stream, h->d, h->sf.U, h->sf.V, h->sf.W, f.dz, h->d_partial);
-        HIPCHK(h, hipMemcpyAsync(h->h_partial.data(), h->d_partial, sizeof(double) * 2 * nb,
-                                 hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(h, hipStreamSynchronize(h->stream));
-        double cnt = 0, mx = 0;
-        for (int b = 0; b < nb; b++) {
-            cnt += h->h_partial[2 * b];
-            mx = mx > h->h_partial[2 * b + 1] ? mx : h->h_partial[2 * b + 1];
-        }
-        if (cnt > 0)
-            fprintf(stderr,
-                    "icar_adv: winds are not balanced on entry to advect: %.0f cells with "
-                    "|div| > 1e-3 (max %g)\n",
-                    cnt, mx);
+        rc = divergence_check(h);
+        if (rc) return rc;
     }
     if (h->o.rk3) return 0;  // advection_driver.f90:112-115
-    if (h->nspecies == 0) return 0;
-    if (h->o.kernel_variant == 1) {
-        rc = fast_advect(h->fast, h->stream, h->d, h->sf, f, h->o.mpdata_order, h->o.mpdata_fct,
-                         [&](int id, double bytes, const std::function<void()> &fn) {
-                             timed(h, id, bytes, fn);
-                         });
-        if (rc) return set_err(h, rc, "%s", fast_error(h->fast));
-        HIPCHK(h, hipGetLastError());
-        return 0;
-    }
     for (int s = 0; s < h->nspecies; s++) {
         if (!h->sp[s].bound) continue;
         float *q = nullptr;
@@ -335,8 +371,32 @@ int sync_fast_species(Handle *h)
         h->compact[s] = (int)v.size();
         v.push_back({h->sp[s].dev, h->sp[s].storage, h->sp[s].pow2});
     }
+    fast_batch_destroy(h->batch);  // its tables point at the old buffers
+    h->batch = nullptr;
     int rc = fast_set_species(h->fast, v.data(), (int)v.size());
     if (rc) return set_err(h, rc, "%s", fast_error(h->fast));
+    return 0;
+}
+
+// Ring conditions for every species of h, launched on stream s and timed
+// by owner (the handle itself, or an ensemble).
+template <class T>
+int apply_bc_on(Handle *h, T *owner, hipStream_t stream, int bc_x, int bc_y)
+{
+    if (bc_x < 0 || bc_x > 2 || (bc_y != 0 && bc_y != 2))
+        return set_err(h, ICAR_ADV_EINVAL, "bc_x must be 0/1/2 and bc_y 0/2");
+    if (bc_x == 0 && bc_y == 0) return 0;
+    for (int s = 0; s < h->nspecies; s++) {
+        SpeciesSlot &sl = h->sp[s];
+        if (!sl.bound) continue;
+        timed(owner, K_BC, 0.0, [&] {
+            if (sl.storage == ICAR_ADV_STORE_F32)
+                launch_bc(stream, h->d, static_cast<float *>(species_dev(h, s)), bc_x, bc_y);
+            else
+                launch_bc16(stream, h->d, species_dev(h, s), bc_x, bc_y);
+        });
+    }
+    HIPCHK(h, hipGetLastError());
     return 0;
 }
 
@@ -621,21 +681,7 @@ int icar_adv_apply_bc(void *handle, int bc_x, int bc_y)
     Handle *h;
     int rc = check_handle(handle, &h);
     if (rc) return rc;
-    if (bc_x < 0 || bc_x > 2 || (bc_y != 0 && bc_y != 2))
-        return set_err(h, ICAR_ADV_EINVAL, "bc_x must be 0/1/2 and bc_y 0/2");
-    if (bc_x == 0 && bc_y == 0) return 0;
-    for (int s = 0; s < h->nspecies; s++) {
-        SpeciesSlot &sl = h->sp[s];
-        if (!sl.bound) continue;
-        timed(h, K_BC, 0.0, [&] {
-            if (sl.storage == ICAR_ADV_STORE_F32)
-                launch_bc(h->stream, h->d, static_cast<float *>(species_dev(h, s)), bc_x, bc_y);
-            else
-                launch_bc16(h->stream, h->d, species_dev(h, s), bc_x, bc_y);
-        });
-    }
-    HIPCHK(h, hipGetLastError());
-    return 0;
+    return apply_bc_on(h, h, h->stream, bc_x, bc_y);
 }
 
 int icar_adv_run(void *handle, float dt, int nsteps, int bc_x, int bc_y)
@@ -644,12 +690,73 @@ int icar_adv_run(void *handle, float dt, int nsteps, int bc_x, int bc_y)
     int rc = check_handle(handle, &h);
     if (rc) return rc;
     if (nsteps < 0) return set_err(h, ICAR_ADV_EINVAL, "nsteps < 0");
-    for (int s = 0; s < nsteps; s++) {
-        rc = do_advect(h, dt);
-        if (rc) return rc;
-        rc = icar_adv_apply_bc(handle, bc_x, bc_y);
-        if (rc) return rc;
+    auto steps = [&]() -> int {
+        for (int s = 0; s < nsteps; s++) {
+            int r = do_advect(h, dt);
+            if (r) return r;
+            r = icar_adv_apply_bc(handle, bc_x, bc_y);
+            if (r) return r;
+        }
+        return 0;
+    };
+    // graph capture needs every launch to be asynchronous: not with the
+    // profiling events or the debug check's host copies
+    if (!h->use_graph || h->prof || h->o.debug || nsteps == 0) return steps();
+    const bool fused = h->fast && h->o.kernel_variant == 1 && !h->o.rk3 && h->nspecies > 0 &&
+                       h->o.mpdata_order >= 1;
+    const int p0 = h->fast ? fast_parity(h->fast) : 0;
+    if (h->gexec && h->g_dt == dt && h->g_steps == nsteps && h->g_bcx == bc_x &&
+        h->g_bcy == bc_y && h->g_par0 == p0) {
+        if (h->fast) fast_set_parity(h->fast, h->g_par1);
+        HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+        return 0;
     }
+    if (h->gexec) {
+        (void)hipGraphExecDestroy(h->gexec);
+        h->gexec = nullptr;
+    }
+    rc = require_bound(h);
+    if (rc) return rc;
+    if (fused) {  // table uploads happen before the capture
+        rc = ensure_batch(h);
+        if (rc) return rc;
+        std::string e;
+        rc = fast_batch_prepare(h->batch, &dt, &e);
+        if (rc) return set_err(h, rc, "%s", e.c_str());
+    }
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    rc = steps();
+    hipGraph_t g = nullptr;
+    hipError_t ec = hipStreamEndCapture(h->stream, &g);
+    if (rc || ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (h->fast) fast_set_parity(h->fast, p0);
+        if (rc) return rc;
+        return set_err(h, (int)ec, "hipStreamEndCapture failed: %s", hipGetErrorString(ec));
+    }
+    ec = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ec != hipSuccess) {
+        h->gexec = nullptr;
+        if (h->fast) fast_set_parity(h->fast, p0);
+        return set_err(h, (int)ec, "hipGraphInstantiate failed: %s", hipGetErrorString(ec));
+    }
+    h->g_dt = dt;
+    h->g_steps = nsteps;
+    h->g_bcx = bc_x;
+    h->g_bcy = bc_y;
+    h->g_par0 = p0;
+    h->g_par1 = h->fast ? fast_parity(h->fast) : 0;  // advanced while recording
+    HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+    return 0;
+}
+
+int icar_adv_set_graph(void *handle, int enable)
+{
+    Handle *h;
+    int rc = check_handle(handle, &h);
+    if (rc) return rc;
+    h->use_graph = enable != 0;
     return 0;
 }
 
@@ -691,6 +798,224 @@ int icar_adv_kernel_stats(void *handle, int id, double *total_ms, long long *lau
     if (total_ms) *total_ms = h->acc_ms[id];
     if (launches) *launches = h->acc_n[id];
     if (alg_bytes) *alg_bytes = h->acc_bytes[id];
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Ensembles: independent domains advanced by one launch per pass
+// ---------------------------------------------------------------------------
+namespace {
+struct Ensemble {
+    std::vector<Handle *> hs;
+    FastBatch *batch = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool prof = false;
+    std::vector<TimedLaunch> pending;
+    std::vector<hipEvent_t> event_pool;
+    double acc_ms[K_COUNT] = {};
+    long long acc_n[K_COUNT] = {};
+    double acc_bytes[K_COUNT] = {};
+    hipGraphExec_t gexec = nullptr;
+    std::vector<float> g_dt;
+    int g_steps = -1, g_bcx = -1, g_bcy = -1, g_par0 = -1, g_par1 = -1;
+};
+
+int ens_err(Ensemble *e, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (e)
+        e->err = buf;
+    else
+        g_create_error = buf;
+    return code;
+}
+
+void ens_free(Ensemble *e)
+{
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    drain_profile(e);
+    for (auto ev : e->event_pool) (void)hipEventDestroy(ev);
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    fast_batch_destroy(e->batch);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+int ens_steps(Ensemble *e, const float *dt, int nsteps, int bc_x, int bc_y)
+{
+    for (int st = 0; st < nsteps; st++) {
+        std::string m;
+        int rc = fast_batch_advect(e->batch, e->stream, dt, timed_fn(e), &m);
+        if (rc) return ens_err(e, rc, "%s", m.c_str());
+        for (size_t k = 0; k < e->hs.size(); k++) {
+            Handle *h = e->hs[k];
+            h->last_dt = dt[k];
+            h->have_dt = true;
+            rc = apply_bc_on(h, e, e->stream, bc_x, bc_y);
+            if (rc) return ens_err(e, rc, "%s", h->err.c_str());
+        }
+    }
+    hipError_t x = hipGetLastError();
+    if (x != hipSuccess) return ens_err(e, (int)x, "launch failed: %s", hipGetErrorString(x));
+    return 0;
+}
+
+void ens_set_parity(Ensemble *e, int p)
+{
+    for (Handle *h : e->hs) fast_set_parity(h->fast, p);
+}
+}  // namespace
+
+extern "C" {
+
+int icar_adv_ensemble_create(int n, void *const *handles, void **ens)
+{
+    if (!ens || !handles || n < 1) return ens_err(nullptr, ICAR_ADV_EINVAL, "bad ensemble arguments");
+    *ens = nullptr;
+    std::vector<FastDomain> doms;
+    Ensemble *e = new Ensemble();
+    for (int k = 0; k < n; k++) {
+        Handle *h = static_cast<Handle *>(handles[k]);
+        if (!h) {
+            ens_free(e);
+            return ens_err(nullptr, ICAR_ADV_EINVAL, "NULL handle %d", k);
+        }
+        if (h->o.kernel_variant != 1 || h->o.rk3 || h->o.mpdata_order < 1 || h->nspecies == 0 ||
+            require_bound(h) != 0) {
+            ens_free(e);
+            return ens_err(nullptr, ICAR_ADV_EINVAL,
+                           "handle %d: ensembles need kernel_variant 1, RK3 off, mpdata_order >= 1, "
+                           "bound fields and species",
+                           k);
+        }
+        if (h->o.mpdata_order != static_cast<Handle *>(handles[0])->o.mpdata_order ||
+            h->o.mpdata_fct != static_cast<Handle *>(handles[0])->o.mpdata_fct) {
+            ens_free(e);
+            return ens_err(nullptr, ICAR_ADV_EINVAL,
+                           "all domains of an ensemble need the same mpdata_order and MPDATA_FCT");
+        }
+        (void)hipStreamSynchronize(h->stream);
+        e->hs.push_back(h);
+        doms.push_back({h->fast, domain_fields(h), h->sf, h->g.dx, h->o.advect_density});
+    }
+    std::string m;
+    e->batch = fast_batch_create(doms.data(), n, e->hs[0]->o.mpdata_order, e->hs[0]->o.mpdata_fct, &m);
+    if (!e->batch) {
+        ens_free(e);
+        return ens_err(nullptr, ICAR_ADV_EINVAL, "%s", m.c_str());
+    }
+    hipError_t x = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (x != hipSuccess) {
+        e->stream = nullptr;
+        ens_free(e);
+        return ens_err(nullptr, (int)x, "hipStreamCreate failed: %s", hipGetErrorString(x));
+    }
+    *ens = e;
+    return 0;
+}
+
+int icar_adv_ensemble_destroy(void *ens)
+{
+    ens_free(static_cast<Ensemble *>(ens));
+    return 0;
+}
+
+int icar_adv_ensemble_run(void *ens, const float *dt, int nsteps, int bc_x, int bc_y, int use_graph)
+{
+    Ensemble *e = static_cast<Ensemble *>(ens);
+    if (!e) return ens_err(nullptr, ICAR_ADV_EINVAL, "NULL ensemble");
+    if (!dt || nsteps < 0) return ens_err(e, ICAR_ADV_EINVAL, "bad dt or nsteps");
+    for (Handle *h : e->hs) (void)hipStreamSynchronize(h->stream);
+    const int n = (int)e->hs.size();
+    if (!use_graph || e->prof || nsteps == 0) return ens_steps(e, dt, nsteps, bc_x, bc_y);
+    const int p0 = fast_parity(e->hs[0]->fast);
+    bool hit = e->gexec && e->g_steps == nsteps && e->g_bcx == bc_x && e->g_bcy == bc_y &&
+               e->g_par0 == p0 && (int)e->g_dt.size() == n;
+    for (int k = 0; hit && k < n; k++) hit = e->g_dt[k] == dt[k];
+    if (hit) {
+        ens_set_parity(e, e->g_par1);
+        hipError_t x = hipGraphLaunch(e->gexec, e->stream);
+        return x == hipSuccess ? 0 : ens_err(e, (int)x, "hipGraphLaunch: %s", hipGetErrorString(x));
+    }
+    if (e->gexec) {
+        (void)hipGraphExecDestroy(e->gexec);
+        e->gexec = nullptr;
+    }
+    std::string m;
+    int rc = fast_batch_prepare(e->batch, dt, &m);
+    if (rc) return ens_err(e, rc, "%s", m.c_str());
+    hipError_t x = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+    if (x != hipSuccess) return ens_err(e, (int)x, "hipStreamBeginCapture: %s", hipGetErrorString(x));
+    rc = ens_steps(e, dt, nsteps, bc_x, bc_y);
+    hipGraph_t g = nullptr;
+    x = hipStreamEndCapture(e->stream, &g);
+    if (rc || x != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        ens_set_parity(e, p0);
+        return rc ? rc : ens_err(e, (int)x, "hipStreamEndCapture: %s", hipGetErrorString(x));
+    }
+    x = hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (x != hipSuccess) {
+        e->gexec = nullptr;
+        ens_set_parity(e, p0);
+        return ens_err(e, (int)x, "hipGraphInstantiate: %s", hipGetErrorString(x));
+    }
+    e->g_dt.assign(dt, dt + n);
+    e->g_steps = nsteps;
+    e->g_bcx = bc_x;
+    e->g_bcy = bc_y;
+    e->g_par0 = p0;
+    e->g_par1 = fast_parity(e->hs[0]->fast);
+    x = hipGraphLaunch(e->gexec, e->stream);
+    return x == hipSuccess ? 0 : ens_err(e, (int)x, "hipGraphLaunch: %s", hipGetErrorString(x));
+}
+
+int icar_adv_ensemble_sync(void *ens)
+{
+    Ensemble *e = static_cast<Ensemble *>(ens);
+    if (!e) return ens_err(nullptr, ICAR_ADV_EINVAL, "NULL ensemble");
+    hipError_t x = hipStreamSynchronize(e->stream);
+    return x == hipSuccess ? 0 : ens_err(e, (int)x, "hipStreamSynchronize: %s", hipGetErrorString(x));
+}
+
+const char *icar_adv_ensemble_last_error(void *ens)
+{
+    if (!ens) return g_create_error.c_str();
+    return static_cast<Ensemble *>(ens)->err.c_str();
+}
+
+int icar_adv_ensemble_profile(void *ens, int enable)
+{
+    Ensemble *e = static_cast<Ensemble *>(ens);
+    if (!e) return ens_err(nullptr, ICAR_ADV_EINVAL, "NULL ensemble");
+    (void)hipStreamSynchronize(e->stream);
+    drain_profile(e);
+    if (enable)
+        for (int k = 0; k < K_COUNT; k++) e->acc_ms[k] = e->acc_n[k] = e->acc_bytes[k] = 0;
+    e->prof = enable != 0;
+    return 0;
+}
+
+int icar_adv_ensemble_kernel_stats(void *ens, int id, double *total_ms, long long *launches,
+                                   double *alg_bytes)
+{
+    Ensemble *e = static_cast<Ensemble *>(ens);
+    if (!e) return ens_err(nullptr, ICAR_ADV_EINVAL, "NULL ensemble");
+    if (id < 0 || id >= K_COUNT) return ens_err(e, ICAR_ADV_EINVAL, "unknown kernel id %d", id);
+    (void)hipStreamSynchronize(e->stream);
+    drain_profile(e);
+    if (total_ms) *total_ms = e->acc_ms[id];
+    if (launches) *launches = e->acc_n[id];
+    if (alg_bytes) *alg_bytes = e->acc_bytes[id];
     return 0;
 }
 

@@ -2,6 +2,7 @@
 // precision species storage it supports.
 #pragma once
 #include <functional>
+#include <string>
 
 #include "common.hpp"
 
@@ -22,9 +23,30 @@ const char *fast_error(FastState *st);
 // Device buffer holding the current state of species s (in bind order).
 void *fast_species_ptr(FastState *st, int s);
 
+// Current home-buffer parity of a domain (alternates each step for
+// mpdata_order <= 2).
+int fast_parity(FastState *st);
+void fast_set_parity(FastState *st, int parity);
+
+// A launch group: one domain (a handle) or an ensemble of domains advanced
+// by one launch per pass.
+struct FastDomain {
+    FastState *st;
+    DomainFields f;
+    StepFields sf;
+    float dx;
+    int advect_density;
+};
+struct FastBatch;
+FastBatch *fast_batch_create(const FastDomain *doms, int n, int order, int fct, std::string *err);
+void fast_batch_destroy(FastBatch *b);
+// Uploads the domain tables for these time steps (one per domain) if they
+// changed; fast_batch_advect calls it, graph capture calls it beforehand.
+int fast_batch_prepare(FastBatch *b, const float *dt, std::string *err);
+
 using TimedFn = std::function<void(int id, double bytes, const std::function<void()> &)>;
-int fast_advect(FastState *st, hipStream_t s, Dims d, const StepFields &sf, const DomainFields &f,
-                int order, int fct, const TimedFn &timed);
+int fast_batch_advect(FastBatch *b, hipStream_t s, const float *dt, const TimedFn &timed,
+                      std::string *err);
 
 // FP32 <-> scaled BF16/FP16 conversion and ring conditions for 16-bit storage.
 void launch_pack(hipStream_t s, size_t n, const float *src, void *dst, int storage, int pow2);

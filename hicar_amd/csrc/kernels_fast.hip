@@ -12,6 +12,7 @@
 //              the 1-D FCT limiter in x, y and z (flux_limiter, :254-350 and
 //              adv_mpdata_FCT_core.f90) and the donor-cell flux divergence
 //              with the limited velocities, fused in one kernel.
+// k_courant_b mpdata_compute_wind for every domain of the batch
 // k_corr marches upward through the levels of a 60 x 16 column tile (halo 2,
 // 64 x 20 columns staged in LDS); x/y neighbours come from LDS planes, the z
 // direction (including the column-wise FCT) stays in registers of the thread
@@ -85,6 +86,102 @@ struct SpecPass {
     float scale0, inv_scaleo;
 };
 
+// One domain of a batched launch (a single handle, or one member of an
+// ensemble of unevenly sized domains).  Cell kernels cover 64x4-thread
+// blocks of one level; the corrective kernel covers (tile, species) items.
+constexpr int MAXP = 16;  // mpdata_order limit
+struct DomDesc {
+    Dims d;
+    unsigned cb0, bx;     // first global cell block; cell blocks along x
+    unsigned w0;          // first global corrective work item
+    int tiles_x, nspec;
+    float dx, dt;
+    int advect_density;
+    DomainFields f;       // raw inputs
+    StepFields sf;        // Courant numbers and products (written by k_courant_b)
+    float *coef;          // face coefficients (float4 groups)
+    const SpecPass *pass[MAXP];  // species table of pass p at index p-1
+};
+
+// domain of global cell block b / work item w: the last entry whose first
+// block (item) is <= the key
+__device__ __forceinline__ int find_dom_cb(const DomDesc *D, int nd, unsigned b)
+{
+    int lo = 0, hi = nd - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (D[mid].cb0 <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int find_dom_w(const DomDesc *D, int nd, unsigned w)
+{
+    int lo = 0, hi = nd - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (D[mid].w0 <= w) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// (i, k, j) of this thread in a 64x4 cell block
+__device__ __forceinline__ void cell_of(const DomDesc &D, unsigned b, int &i, int &k, int &j)
+{
+    const unsigned l = b - D.cb0;
+    const unsigned r = l / D.bx;
+    i = (int)(l % D.bx) * 64 + (int)threadIdx.x;
+    k = (int)(r % (unsigned)D.d.nz);
+    j = (int)(r / (unsigned)D.d.nz) * 4 + (int)threadIdx.y;
+}
+
+// ---------------------------------------------------------------------------
+// mpdata_compute_wind (adv_mpdata.f90:477-504) and the species-invariant
+// products, for every domain of the batch; the same expressions as the
+// exact Courant kernel, so the same bits.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_courant_b(const DomDesc *__restrict__ DD, int nd)
+{
+    const int dm = find_dom_cb(DD, nd, blockIdx.x);
+    const DomDesc &D = DD[dm];
+    int i, k, j;
+    cell_of(D, blockIdx.x, i, k, j);
+    const Dims d = D.d;
+    if (i >= d.nx || j >= d.ny) return;
+    const DomainFields &f = D.f;
+    const StepFields &o = D.sf;
+    const float dt = D.dt, dx = D.dx;
+    const int advect_density = D.advect_density;
+    const size_t c = ix(d, i, k, j);
+    const size_t sj = (size_t)d.nx * d.nz;
+    const float rc = advect_density ? f.density[c] : 1.0f;  // :489-490
+    if (i >= 1) {
+        const size_t iu = (size_t)i + (size_t)(d.nx + 1) * ((size_t)k + (size_t)d.nz * j);
+        const float rl = advect_density ? f.density[c - 1] : 1.0f;
+        o.U[c] = f.u[iu] * dt * (rc + rl) * 0.5f * f.jaco_u[iu] / dx;  // :492-493
+    } else {
+        o.U[c] = 0.0f;
+    }
+    if (j >= 1) {
+        const float rs = advect_density ? f.density[c - sj] : 1.0f;
+        o.V[c] = f.v[c] * dt * (rc + rs) * 0.5f * f.jaco_v[c] / dx;  // :494-495
+    } else {
+        o.V[c] = 0.0f;
+    }
+    float Wc;
+    if (k < d.nz - 1) {
+        const float ru = advect_density ? f.density[c + d.nx] : 1.0f;
+        Wc = f.w[c] * dt * f.jaco_w[c] * (ru + rc) * 0.5f;  // :496-497
+    } else {
+        Wc = f.w[c] * dt * f.jaco_w[c] * rc;  // :498
+    }
+    o.W[c] = Wc;
+    const float jac = f.jaco[c], den = f.density[c], dzc = f.dz[c];
+    o.G[c] = jac * den;
+    o.Gz[c] = dzc * jac * den;
+    o.Wd[c] = Wc / dzc;
+}
+
 // ---------------------------------------------------------------------------
 // species-invariant parts of the pseudo-velocities, once per step, each
 // evaluated exactly as mpdata_fluxes evaluates it (adv_mpdata.f90:139,151,
@@ -101,14 +198,17 @@ struct SpecPass {
 // ---------------------------------------------------------------------------
 enum { CUX = 0, CXV, CXW, SGX, CUY, CYU, CYW, SGY, CUZ, CZU, CZV, SGZ, NCOEF };
 
-__global__ void __launch_bounds__(256)
-    k_coef(Dims d, const float *__restrict__ U, const float *__restrict__ V,
-           const float *__restrict__ Wd, const float *__restrict__ G, float *__restrict__ coef)
+__global__ void __launch_bounds__(256) k_coef(const DomDesc *__restrict__ DD, int nd)
 {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.z * 4 + threadIdx.y;
-    const int k = blockIdx.y;
+    const int dm = find_dom_cb(DD, nd, blockIdx.x);
+    const DomDesc &D = DD[dm];
+    int i, k, j;
+    cell_of(D, blockIdx.x, i, k, j);
+    const Dims d = D.d;
     if (i >= d.nx || j >= d.ny) return;
+    const float *__restrict__ U = D.sf.U, *__restrict__ V = D.sf.V, *__restrict__ Wd = D.sf.Wd;
+    const float *__restrict__ G = D.sf.G;
+    float *__restrict__ coef = D.coef;
     const int nx = d.nx, ny = d.ny, nz = d.nz;
     const size_t N = (size_t)nx * ny * nz;
     const size_t sk = (size_t)nx, sj = (size_t)nx * nz;
@@ -160,15 +260,18 @@ __device__ __forceinline__ float flux1(float l, float r, float U)
     return ((U + fabsf(U)) * l + (U - fabsf(U)) * r) / 2.0f;
 }
 
-__global__ void __launch_bounds__(FBX *FBY)
-    k_first(Dims d, const float *__restrict__ U, const float *__restrict__ V,
-            const float *__restrict__ W, const float *__restrict__ G,
-            const float *__restrict__ Gz, const SpecPass *__restrict__ sp, int nspec)
+__global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ DD, int nd)
 {
-    const int i = blockIdx.x * FBX + threadIdx.x;
-    const int j = blockIdx.z * FBY + threadIdx.y;
-    const int k = blockIdx.y;
+    const int dm = find_dom_cb(DD, nd, blockIdx.x);
+    const DomDesc &D = DD[dm];
+    int i, k, j;
+    cell_of(D, blockIdx.x, i, k, j);
+    const Dims d = D.d;
     if (i >= d.nx || j >= d.ny) return;
+    const float *__restrict__ U = D.sf.U, *__restrict__ V = D.sf.V, *__restrict__ W = D.sf.W;
+    const float *__restrict__ G = D.sf.G, *__restrict__ Gz = D.sf.Gz;
+    const SpecPass *__restrict__ sp = D.pass[0];
+    const int nspec = D.nspec;
     const size_t sk = (size_t)d.nx, sj = (size_t)d.nx * d.nz;
     const size_t c = (size_t)i + sk * ((size_t)k + (size_t)d.nz * j);
     const bool ring = i == 0 || i == d.nx - 1 || j == 0 || j == d.ny - 1;
@@ -225,30 +328,26 @@ constexpr int CNT = 64 * NW;
 constexpr int SR = 3;             // staged-row slots per wave (the third only for ly < 4)
 constexpr int OWN = 2;            // output rows per wave: ly + 8*(rb + t), t = 0, 1
 
-struct CorrArgs {
-    Dims d;
-    const float *coef;  // NCOEF planes of N
-    const float *G, *Gz, *dz;
-    const SpecPass *sp;
-    int nspec, tiles_x, nwork;
-};
 
 // WPE: waves per SIMD the register allocation targets (2 blocks of 8 waves
 // per CU need 4); PF: load this iteration's coefficients before the first
 // barrier (latency hidden by the barrier) or at their use.
 template <bool FCT, int WPE, bool PF>
-__global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) k_corr(CorrArgs a)
+__global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
+    k_corr(const DomDesc *__restrict__ DD, int nd, int pass, unsigned nwork)
 {
     // XCD-aware mapping: consecutive work items (the species of one tile)
     // land on the same XCD (blocks are dealt round-robin over 8 XCDs) and
     // share the tile's coefficients in its L2.
-    const int nb = gridDim.x;
-    const int w = (blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3);
-    if (w >= a.nwork) return;
-    const int s = w % a.nspec;
-    const int tile = w / a.nspec;
-    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
-    const int nx = a.d.nx, ny = a.d.ny, nz = a.d.nz;
+    const unsigned nb = gridDim.x;
+    const unsigned w = (blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3);
+    if (w >= nwork) return;
+    const DomDesc &D = DD[find_dom_w(DD, nd, w)];
+    const unsigned lw = w - D.w0;
+    const int s = (int)(lw % (unsigned)D.nspec);
+    const int tile = (int)(lw / (unsigned)D.nspec);
+    const int tx = tile % D.tiles_x, ty = tile / D.tiles_x;
+    const int nx = D.d.nx, ny = D.d.ny, nz = D.d.nz;
     // 32-bit element offsets (the host checks nx*ny*nz < 2^32) so that the
     // loads use a scalar base and one VGPR offset
     const unsigned sk = (unsigned)nx, sj = (unsigned)nx * (unsigned)nz;
@@ -256,10 +355,11 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
     const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
     const int gi = tx * CTI - CH + lx;
     const int j0 = ty * CTJ - CH;  // global row of staged row 0
-    const SpecPass p = a.sp[s];
+    const SpecPass p = D.pass[pass - 1][s];
     const bool same = p.q1 == nullptr;
     const float eps = 1e-10f, eps15 = 1e-15f;
-    const float4 *__restrict__ gx = reinterpret_cast<const float4 *>(a.coef);
+    const float *__restrict__ Gp = D.sf.G, *__restrict__ Gzp = D.sf.Gz, *__restrict__ dzp = D.f.dz;
+    const float4 *__restrict__ gx = reinterpret_cast<const float4 *>(D.coef);
     const float4 *__restrict__ gy = gx + N;
     const float4 *__restrict__ gz = gx + 2 * N;
 
@@ -355,8 +455,8 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                         kx[t][1] = v.y;
                         kx[t][2] = v.z;
                         kx[t][3] = v.w;
-                        kg[t][0] = a.G[c];
-                        kg[t][1] = a.Gz[c];
+                        kg[t][0] = Gp[c];
+                        kg[t][1] = Gzp[c];
                     }
                     if (L1 >= 0 && L1 <= nz - 2) {
                         const unsigned c = rowbase[r] + sk * (unsigned)L1;
@@ -365,7 +465,7 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                         kz[t][1] = v.y;
                         kz[t][2] = v.z;
                         kz[t][3] = v.w;
-                        kz[t][4] = a.dz[c];
+                        kz[t][4] = dzp[c];
                     }
                 }
             }
@@ -662,15 +762,27 @@ struct FastState {
     Dims d{};
     size_t N = 0;
     std::string err;
-    std::vector<FastSpecies> sp;      // home buffers as bound
-    std::vector<void *> alt;          // second home buffer per species
-    int cur = 0;                      // which home holds the current state
+    std::vector<FastSpecies> sp;         // home buffers as bound
+    std::vector<void *> alt;             // second home buffer per species
+    int cur = 0;                         // which home holds the current state
+    int gen = 0;                         // bumped whenever the species buffers change
     float *s1 = nullptr, *s2 = nullptr;  // FP32 scratch, nspec * N each
-    float *coef = nullptr;            // NCOEF planes of N
-    SpecPass *d_tab = nullptr;        // [2 parities][max passes][nspec]
+    float *coef = nullptr;               // NCOEF values per cell (float4 groups)
+    SpecPass *d_tab = nullptr;           // [2 parities][passes][nspec]
     int tab_order = -1;
     size_t tab_cap = 0;
-    int corr_cfg = 1;  // corrective-kernel configuration: bit 0 WPE 4, bit 1 no prefetch
+};
+
+struct FastBatch {
+    std::vector<FastDomain> doms;
+    std::vector<int> gens;
+    int order = 0, fct = 0;
+    DomDesc *d_desc = nullptr;           // [2 parities][n]
+    std::vector<DomDesc> h_desc[2];
+    std::vector<float> dts;              // dt values in the device tables
+    unsigned total_cb = 0, total_work = 0;
+    double cells = 0, cell_species = 0;  // for the algorithmic byte counts
+    int corr_cfg = 1;  // corrective kernel: bit 0 -> 4 waves/SIMD, bit 1 -> no prefetch
 };
 
 namespace {
@@ -691,7 +803,7 @@ void *home(FastState *st, int s, int which)
     return which == 0 ? st->sp[s].dev : st->alt[s];
 }
 
-// Builds the per-pass species tables for both home parities.
+// Per-pass species tables of one domain for both home parities.
 int build_tables(FastState *st, int order)
 {
     const int S = (int)st->sp.size();
@@ -716,17 +828,15 @@ int build_tables(FastState *st, int order)
                 float *sa = st->s1 + (size_t)s * st->N, *sb = st->s2 + (size_t)s * st->N;
                 SpecPass e{};
                 void *hcur = home(st, s, par), *hoth = home(st, s, 1 - par);
-                // input of this pass
-                if (pass == 1) {
+                if (pass == 1) {  // q0 = home
                     e.q0 = hcur, e.kind0 = f.storage, e.scale0 = sc, e.q1 = nullptr;
-                } else if (pass == 2) {
+                } else if (pass == 2) {  // q0 = home, q1 = first pass
                     e.q0 = hcur, e.kind0 = f.storage, e.scale0 = sc, e.q1 = sa;
-                } else {
-                    e.q0 = (pass % 2 == 1) ? sb : sa;  // output of the previous pass
+                } else {  // q0 = q1 = previous pass (adv_mpdata.f90:388-397)
+                    e.q0 = (pass % 2 == 1) ? sb : sa;
                     e.kind0 = ICAR_ADV_STORE_F32, e.scale0 = 1.0f, e.q1 = nullptr;
                 }
-                // output of this pass
-                if (pass == P) {
+                if (pass == P) {  // home is not read after pass 2
                     e.out = (P <= 2) ? hoth : hcur;
                     e.kindo = f.storage, e.inv_scaleo = isc;
                 } else {
@@ -744,13 +854,17 @@ int build_tables(FastState *st, int order)
     return 0;
 }
 
+unsigned cell_blocks(const Dims &d)
+{
+    return (unsigned)((d.nx + 63) / 64) * (unsigned)d.nz * (unsigned)((d.ny + 3) / 4);
+}
+
 }  // namespace
 
 FastState *fast_create(Dims d)
 {
     FastState *st = new FastState();
     st->d = d;
-    if (const char *e = getenv("ICAR_ADV_CORR_CFG")) st->corr_cfg = atoi(e) & 3;
     st->N = (size_t)d.nx * d.ny * d.nz;
     if (hipMalloc(&st->coef, (size_t)NCOEF * st->N * sizeof(float)) != hipSuccess) {
         fast_destroy(st);
@@ -774,6 +888,7 @@ int fast_set_species(FastState *st, const FastSpecies *sp, int n)
     st->sp.assign(sp, sp + n);
     st->cur = 0;
     st->tab_order = -1;
+    st->gen++;
     if (n == 0) return 0;
     st->alt.assign(n, nullptr);
     for (int s = 0; s < n; s++)
@@ -795,65 +910,157 @@ void *fast_species_ptr(FastState *st, int s)
     return home(st, s, st->cur);
 }
 
+int fast_parity(FastState *st) { return st ? st->cur : 0; }
+
+void fast_set_parity(FastState *st, int parity)
+{
+    if (st) st->cur = parity & 1;
+}
+
 const char *fast_error(FastState *st) { return st ? st->err.c_str() : "no fused state"; }
 
-int fast_advect(FastState *st, hipStream_t s, Dims d, const StepFields &sf, const DomainFields &f,
-                int order, int fct, const TimedFn &timed)
+FastBatch *fast_batch_create(const FastDomain *doms, int n, int order, int fct, std::string *err)
 {
-    const int S = (int)st->sp.size();
-    if (S == 0 || order < 1) return 0;
-    if (st->tab_order != order) {
-        int rc = build_tables(st, order);
-        if (rc) return rc;
+    if (n < 1 || order < 1 || order > MAXP) {
+        *err = "a batch needs at least one domain and 1 <= mpdata_order <= 16";
+        return nullptr;
     }
-    const int P = order;
-    const double n = (double)st->N;
-    const SpecPass *tab = st->d_tab + (size_t)st->cur * P * S;
-    const dim3 cgrid((d.nx + 63) / 64, d.nz, (d.ny + 3) / 4);
-    // species-invariant coefficients: reads U, V, Wd, G, Gz, dz; writes NCOEF planes
-    timed(K_COEF, (4.0 + NCOEF) * 4 * n, [&] {
-        hipLaunchKernelGGL(k_coef, cgrid, dim3(64, 4), 0, s, d, sf.U, sf.V, sf.Wd, sf.G, st->coef);
-    });
+    FastBatch *b = new FastBatch();
+    b->doms.assign(doms, doms + n);
+    b->order = order;
+    b->fct = fct;
+    if (const char *e = getenv("ICAR_ADV_CORR_CFG")) b->corr_cfg = atoi(e) & 3;
+    unsigned cb = 0, w = 0;
+    for (int i = 0; i < 2; i++) b->h_desc[i].resize(n);
+    for (int k = 0; k < n; k++) {
+        FastState *st = doms[k].st;
+        if (st->N >= 4294967296ull) {
+            *err = "the fused path needs nx*ny*nz < 2^32 per domain";
+            delete b;
+            return nullptr;
+        }
+        if (st->sp.empty()) {
+            *err = "a domain of the batch has no species bound";
+            delete b;
+            return nullptr;
+        }
+        if (st->tab_order != order && build_tables(st, order)) {
+            *err = st->err;
+            delete b;
+            return nullptr;
+        }
+        b->gens.push_back(st->gen);
+        const Dims &d = st->d;
+        const int S = (int)st->sp.size();
+        DomDesc D{};
+        D.d = d;
+        D.cb0 = cb;
+        D.bx = (unsigned)((d.nx + 63) / 64);
+        D.w0 = w;
+        D.tiles_x = (d.nx + CTI - 1) / CTI;
+        D.nspec = S;
+        D.dx = doms[k].dx;
+        D.dt = 0.0f;
+        D.advect_density = doms[k].advect_density;
+        D.f = doms[k].f;
+        D.sf = doms[k].sf;
+        D.coef = st->coef;
+        const int tiles_y = (d.ny + CTJ - 1) / CTJ;
+        cb += cell_blocks(d);
+        w += (unsigned)(D.tiles_x * tiles_y * S);
+        b->cells += (double)st->N;
+        b->cell_species += (double)st->N * S;
+        for (int par = 0; par < 2; par++) {
+            DomDesc E = D;
+            for (int pass = 1; pass <= order; pass++)
+                E.pass[pass - 1] = st->d_tab + ((size_t)par * order + (pass - 1)) * S;
+            b->h_desc[par][k] = E;
+        }
+    }
+    b->total_cb = cb;
+    b->total_work = w;
+    if (hipMalloc(&b->d_desc, 2 * (size_t)n * sizeof(DomDesc)) != hipSuccess) {
+        *err = "hipMalloc of the domain table failed";
+        delete b;
+        return nullptr;
+    }
+    return b;
+}
+
+void fast_batch_destroy(FastBatch *b)
+{
+    if (!b) return;
+    (void)hipFree(b->d_desc);
+    delete b;
+}
+
+int fast_batch_prepare(FastBatch *b, const float *dt, std::string *err)
+{
+    const int n = (int)b->doms.size();
+    for (int k = 0; k < n; k++)
+        if (b->doms[k].st->gen != b->gens[k]) {
+            *err = "species were re-bound after the batch was built";
+            return ICAR_ADV_EINVAL;
+        }
+    bool same = (int)b->dts.size() == n;
+    for (int k = 0; same && k < n; k++) same = b->dts[k] == dt[k];
+    if (same) return 0;
+    b->dts.assign(dt, dt + n);
+    for (int par = 0; par < 2; par++)
+        for (int k = 0; k < n; k++) b->h_desc[par][k].dt = dt[k];
+    for (int par = 0; par < 2; par++)
+        if (hipMemcpy(b->d_desc + (size_t)par * n, b->h_desc[par].data(), n * sizeof(DomDesc),
+                      hipMemcpyHostToDevice) != hipSuccess) {
+            *err = "upload of the domain table failed";
+            return ICAR_ADV_ENOMEM;
+        }
+    return 0;
+}
+
+int fast_batch_advect(FastBatch *b, hipStream_t s, const float *dt, const TimedFn &timed,
+                      std::string *err)
+{
+    int rc = fast_batch_prepare(b, dt, err);
+    if (rc) return rc;
+    const int n = (int)b->doms.size();
+    const int par = b->doms[0].st->cur;
+    for (int k = 1; k < n; k++)
+        if (b->doms[k].st->cur != par) {
+            *err = "the domains of the batch are at different buffer parities";
+            return ICAR_ADV_EINVAL;
+        }
+    const DomDesc *D = b->d_desc + (size_t)par * n;
+    const double cells = b->cells, cs = b->cell_species;
+    // Courant numbers: reads u, v, w, density, jacobians, dz; writes 6 fields
+    timed(K_COURANT, (9.0 + 6.0) * 4 * cells,
+          [&] { hipLaunchKernelGGL(k_courant_b, dim3(b->total_cb), dim3(64, 4), 0, s, D, n); });
+    // species-invariant coefficients: reads U, V, Wd, G; writes NCOEF
+    timed(K_COEF, (4.0 + NCOEF) * 4 * cells,
+          [&] { hipLaunchKernelGGL(k_coef, dim3(b->total_cb), dim3(64, 4), 0, s, D, n); });
     // pass 1: donor cell for every species
-    timed(K_FUSED_FIRST, (5.0 * 4 + S * 2.0 * 4) * n, [&] {
-        dim3 grid((d.nx + FBX - 1) / FBX, d.nz, (d.ny + FBY - 1) / FBY);
-        hipLaunchKernelGGL(k_first, grid, dim3(FBX, FBY), 0, s, d, sf.U, sf.V, sf.W, sf.G, sf.Gz,
-                           tab, S);
-    });
-    // corrective passes
-    CorrArgs a;
-    a.d = d;
-    a.coef = st->coef;
-    a.G = sf.G;
-    a.Gz = sf.Gz;
-    a.dz = f.dz;
-    a.nspec = S;
-    a.tiles_x = (d.nx + CTI - 1) / CTI;
-    if (st->N >= 4294967296ull) {
-        st->err = "the fused path needs nx*ny*nz < 2^32";
-        return ICAR_ADV_ERANGE;
-    }
-    const int tiles_y = (d.ny + CTJ - 1) / CTJ;
-    a.nwork = a.tiles_x * tiles_y * S;
-    const int nb = (a.nwork + 7) / 8 * 8;
-    for (int pass = 2; pass <= P; pass++) {
-        a.sp = tab + (size_t)(pass - 1) * S;
-        // species read (q0 and q1 at pass 2) and written once; coefficients once
-        const double bytes = (pass == 2 ? 3.0 : 2.0) * 4 * n * S + (NCOEF + 3.0) * 4 * n;
+    timed(K_FUSED_FIRST, 5.0 * 4 * cells + 2.0 * 4 * cs,
+          [&] { hipLaunchKernelGGL(k_first, dim3(b->total_cb), dim3(FBX, FBY), 0, s, D, n); });
+    const unsigned nb = (b->total_work + 7) / 8 * 8;
+    for (int pass = 2; pass <= b->order; pass++) {
+        // species read (q0 and q1 at pass 2) and written once; coefficients,
+        // G, Gz and dz once
+        const double bytes = (pass == 2 ? 3.0 : 2.0) * 4 * cs + (NCOEF + 3.0) * 4 * cells;
         timed(K_FUSED_CORR, bytes, [&] {
-            switch ((fct ? 4 : 0) + st->corr_cfg) {
-            case 0: hipLaunchKernelGGL((k_corr<false, 2, true>), dim3(nb), dim3(CNT), 0, s, a); break;
-            case 1: hipLaunchKernelGGL((k_corr<false, 4, true>), dim3(nb), dim3(CNT), 0, s, a); break;
-            case 2: hipLaunchKernelGGL((k_corr<false, 2, false>), dim3(nb), dim3(CNT), 0, s, a); break;
-            case 3: hipLaunchKernelGGL((k_corr<false, 4, false>), dim3(nb), dim3(CNT), 0, s, a); break;
-            case 4: hipLaunchKernelGGL((k_corr<true, 2, true>), dim3(nb), dim3(CNT), 0, s, a); break;
-            case 5: hipLaunchKernelGGL((k_corr<true, 4, true>), dim3(nb), dim3(CNT), 0, s, a); break;
-            case 6: hipLaunchKernelGGL((k_corr<true, 2, false>), dim3(nb), dim3(CNT), 0, s, a); break;
-            default: hipLaunchKernelGGL((k_corr<true, 4, false>), dim3(nb), dim3(CNT), 0, s, a); break;
+            const unsigned nw = b->total_work;
+            switch ((b->fct ? 4 : 0) + b->corr_cfg) {
+            case 0: hipLaunchKernelGGL((k_corr<false, 2, true>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+            case 1: hipLaunchKernelGGL((k_corr<false, 4, true>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+            case 2: hipLaunchKernelGGL((k_corr<false, 2, false>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+            case 3: hipLaunchKernelGGL((k_corr<false, 4, false>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+            case 4: hipLaunchKernelGGL((k_corr<true, 2, true>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+            case 5: hipLaunchKernelGGL((k_corr<true, 4, true>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+            case 6: hipLaunchKernelGGL((k_corr<true, 2, false>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+            default: hipLaunchKernelGGL((k_corr<true, 4, false>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
             }
         });
     }
-    if (P <= 2) st->cur = 1 - st->cur;  // the result landed in the other home buffer
+    if (b->order <= 2)  // the results landed in the other home buffers
+        for (auto &dm : b->doms) dm.st->cur = 1 - dm.st->cur;
     return 0;
 }
 

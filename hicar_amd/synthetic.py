@@ -239,22 +239,28 @@ def config2(nx=1024, ny=1024, nz=30, steps=50, nspecies=10) -> Case:
 C3_SPECIES = ("qv", "qc", "qr", "qs", "qi", "qg")
 
 
-def config3_domain(d: int, steps=100) -> Case:
-    """Domain d of the 48-member ensemble: nx, ny ~ U[64,512], nz in
-    {20,30,40}, its own fractal terrain, C1-type winds, 6 FP32 species,
-    order 2 with FCT (namelist defaults)."""
+def config3_domain(d: int, steps=100, lo: int = 64, hi: int = 512, nzs=(20, 30, 40)) -> Case:
+    """Domain d of the 48-member ensemble: nx, ny ~ U[lo,hi] (64..512), nz
+    from nzs ({20,30,40}), its own fractal terrain, C1-type winds, 6 FP32
+    species, order 2 with FCT (namelist defaults)."""
     r = rng("C3", d, "shape")
-    nx = int(r.integers(64, 513))
-    ny = int(r.integers(64, 513))
-    nz = int((20, 30, 40)[int(r.integers(0, 3))])
+    nx = int(r.integers(lo, hi + 1))
+    ny = int(r.integers(lo, hi + 1))
+    nz = int(nzs[int(r.integers(0, len(nzs)))])
     dom = Domain(nx, ny, nz, 1000.0)
     terrain = fractal_terrain(nx, ny, 0.0, _cap(3000.0, nz), rng("C3", d, "terrain"))
     z = _metric(dom, terrain, dz_levels(nz))
     opts = Options()
     opts.parameters.advect_density = True
+    opts.kernel_variant = 1
     u, v = _ridge_winds(dom, z)
     return _finish(f"C3[{d}]", dom, opts, u, v, 1.2 * np.exp(-z / 8000.0), C3_SPECIES, z, steps,
                    1, 2, "C3", d_index=d)
+
+
+def config3(n: int = 48, steps=100, lo: int = 64, hi: int = 512, nzs=(20, 30, 40)) -> list:
+    """The ensemble: n independent domains of uneven sizes."""
+    return [config3_domain(d, steps, lo, hi, nzs) for d in range(n)]
 
 
 C4_SPECIES = ("qv", "theta", "qc", "qr", "qs", "qi", "qg", "ni", "nr", "ns", "ng", "qtr")

@@ -168,6 +168,29 @@ const char *icar_adv_kernel_name(int id);
 int icar_adv_kernel_stats(void *handle, int id, double *total_ms, long long *launches,
                           double *alg_bytes);
 
+/* Records the nsteps loop of icar_adv_run in a HIP graph on its first call
+ * and replays it on later calls with the same dt, nsteps and boundary
+ * conditions (not while profiling or with debug set). */
+int icar_adv_set_graph(void *handle, int enable);
+
+/* ---- Ensembles ------------------------------------------------------------
+ * No reference counterpart (one ICAR image holds one domain): a set of
+ * independent domains of uneven sizes advanced together, one launch per
+ * pass covering every (domain, tile, species), the step loop optionally
+ * recorded in a HIP graph.  All handles need kernel_variant 1, RK3 off and
+ * the same mpdata_order and MPDATA_FCT; the ensemble uses their device
+ * state, so call icar_adv_ensemble_sync before using a member handle. */
+int icar_adv_ensemble_create(int n, void *const *handles, void **ens);
+int icar_adv_ensemble_destroy(void *ens);
+/* dt holds one time step per domain; bc_x/bc_y as icar_adv_apply_bc. */
+int icar_adv_ensemble_run(void *ens, const float *dt, int nsteps, int bc_x, int bc_y,
+                          int use_graph);
+int icar_adv_ensemble_sync(void *ens);
+const char *icar_adv_ensemble_last_error(void *ens);
+int icar_adv_ensemble_profile(void *ens, int enable);
+int icar_adv_ensemble_kernel_stats(void *ens, int id, double *total_ms, long long *launches,
+                                   double *alg_bytes);
+
 #ifdef __cplusplus
 }
 #endif

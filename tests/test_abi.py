@@ -60,3 +60,18 @@ def test_kernel_names():
     names = [lib.icar_adv_kernel_name(i).decode() for i in range(n)]
     assert "upwind" in names and "fct" in names and len(set(names)) == n
     assert lib.icar_adv_kernel_name(n) == b""
+
+
+def test_fortran_interface_binds_declared_symbols(tmp_path):
+    """Every bind(C) name of the iso_c_binding module is a declared entry
+    point, and the module compiles with the image's flang."""
+    import shutil
+    import subprocess
+    src = os.path.join(ROOT, "fortran", "icar_adv_c.f90")
+    names = set(re.findall(r'bind\(C,\s*name="(icar_adv_\w+)"\)', open(src).read()))
+    assert names and names <= set(declared_symbols())
+    fc = shutil.which("flang") or shutil.which("amdflang")
+    if fc is None:
+        pytest.skip("no Fortran compiler in this environment")
+    subprocess.run([fc, "-c", src, "-module-dir", str(tmp_path), "-o", str(tmp_path / "m.o")],
+                   check=True, timeout=120)
