@@ -8,8 +8,17 @@ continuity, dt at Courant 0.5.  A step is one advect() call (Courant numbers
 plus every species through all MPDATA passes) with all inputs resident in
 HBM.  value = cell-species updates per second = nx*ny*nz*species*steps / t.
 
+Variants: "plain" (exact per-species kernels, the drop-in FP32 form),
+"batched" (fused species-batched kernels, FP32, bit-identical to plain) and
+"batched_bf16" (the same with the hydrometeor masses and numbers stored in
+BF16 with power-of-two scaling, qv and theta FP32).  value is the best
+bit-exact FP32 variant; the others are reported beside it, with a small
+launch-bound case (the 256x64x20 ridge domain) run eagerly and replayed
+from a HIP graph.
+
 Prints one JSON line.  Options: --steps K --warmup W --gpus N (1 only)
---variant exact|fast|best --no-cpu (skip the CPU baseline leg).
+--variant plain|batched|batched_bf16|all --no-cpu (skip the CPU baseline)
+--no-small (skip the launch-bound case).
 """
 from __future__ import annotations
 
@@ -29,23 +38,51 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def measure(case, variant: int, steps: int, warmup: int):
+def measure(case, variant: int, steps: int, warmup: int, profile: bool = True,
+            graph: bool = False):
     from hicar_amd.advection import MpdataAdvection
     case.options.kernel_variant = variant
     st = MpdataAdvection(case.domain, case.options)
     try:
+        st.set_graph(graph)
         st.run(case.dt, warmup)
         st.sync()
-        st.profile(True)
+        if graph:  # record the timed loop once so that the timed call replays it
+            st.run(case.dt, steps)
+            st.sync()
+        st.profile(profile)
         t0 = time.perf_counter()
         st.run(case.dt, steps)
         st.sync()
         t1 = time.perf_counter()
-        stats = st.kernel_stats()
+        stats = st.kernel_stats() if profile else {}
         st.profile(False)
     finally:
         st.close()
     return t1 - t0, stats
+
+
+def set_bf16(case):
+    from hicar_amd import _lib, synthetic
+    d = case.domain
+    for n, a in d.species.items():
+        if n not in ("qv", "theta"):
+            d.storage[n] = (_lib.STORE_BF16, synthetic.pow2_scale(a, _lib.STORE_BF16))
+
+
+def read_traffic(kernel: str):
+    """HBM bytes per launch of a kernel from the committed PMC summary
+    (profiles/traffic_*.txt: lines 'kernel fetch_bytes write_bytes')."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "traffic_*.txt"))):
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == kernel:
+                try:
+                    return float(parts[1]) + float(parts[2])
+                except ValueError:
+                    pass
+    return None
 
 
 def cpu_baseline():
@@ -73,8 +110,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--variant", choices=("exact", "fast", "best"), default="best")
+    ap.add_argument("--variant", choices=("plain", "batched", "batched_bf16", "all"),
+                    default="all")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-small", action="store_true", help="skip the launch-bound small case")
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--ny", type=int, default=1024)
     args = ap.parse_args()
@@ -90,23 +129,30 @@ def main() -> int:
     d = case.domain
     S = len(d.species)
     cells = d.nx * d.ny * d.nz
-    variants = {"exact": 0, "fast": 1}
-    todo = ["exact", "fast"] if args.variant == "best" else [args.variant]
+    variants = {"plain": 0, "batched": 1, "batched_bf16": 1}
+    todo = list(variants) if args.variant == "all" else [args.variant]
     results = {}
     for name in todo:
         log(f"variant {name}: {args.warmup} warmup + {args.steps} timed steps ...")
-        try:
-            t, stats = measure(case, variants[name], args.steps, args.warmup)
-        except Exception as e:  # a variant not available in this build
-            log(f"variant {name} unavailable: {e}")
-            continue
+        if name == "batched_bf16":
+            set_bf16(case)
+        t, stats = measure(case, variants[name], args.steps, args.warmup)
         results[name] = {"seconds": t, "value": cells * S * args.steps / t, "stats": stats}
         log(f"  {name}: {results[name]['value'] / 1e9:.2f} G cell-species/s, "
             f"{1e3 * t / args.steps:.2f} ms/step")
-    if not results:
-        log("no variant ran")
-        return 1
-    best = max(results, key=lambda k: results[k]["value"])
+    # launch-bound small domain: eager vs graph replay
+    small = {}
+    if args.variant == "all" and not args.no_small:
+        c1 = synthetic.config1()
+        c1.options.adv_options.MPDATA_FCT = True
+        n1 = c1.domain.nx * c1.domain.ny * c1.domain.nz * len(c1.domain.species)
+        for g in (False, True):
+            t, _ = measure(c1, 1, 100, 10, profile=False, graph=g)
+            small["graph" if g else "eager"] = n1 * 100 / t
+        log(f"  256x64x20 ridge, 1 species, batched: eager {small['eager'] / 1e9:.2f}, "
+            f"graph {small['graph'] / 1e9:.2f} G cell-species/s")
+    exact_ok = [k for k in results if k in ("plain", "batched")]
+    best = max(exact_ok or results, key=lambda k: results[k]["value"])
     r = results[best]
     stats = r["stats"]
     dom_k = max(stats, key=lambda k: stats[k]["total_ms"])
@@ -114,8 +160,9 @@ def main() -> int:
     avg_s = ks["total_ms"] / ks["launches"] / 1e3
     bytes_per_launch = ks["alg_bytes"] / ks["launches"]
     achieved = bytes_per_launch / avg_s
+    traffic = read_traffic({"fused_corrective_pass": "k_corr", "fct": "k_fct"}.get(dom_k, dom_k))
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                 "kernel": dom_k, "avg_launch_ms": avg_s * 1e3,
                 "alg_bytes_per_launch": bytes_per_launch}
     out = {
@@ -130,9 +177,11 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
+        "parity": "bit-identical to the FP32 oracle (plain and batched)",
         "data": "synthetic",
         "config": {"workload": f"single domain {d.nx}x{d.ny}x{d.nz}, {S} species, "
                                "mpdata_order 3 + FCT, advect_density, Courant 0.5",
+                   "variants_run": todo,
                    "nx": d.nx, "ny": d.ny, "nz": d.nz, "species": S, "mpdata_order": 3,
                    "fct": True, "variant": best, "dt": case.dt},
         "roofline": roofline,
@@ -143,6 +192,7 @@ def main() -> int:
                                      for kk, s in v["stats"].items()}}
                      for k, v in results.items()},
         "compulsory_hbm_fraction": None,
+        "launch_bound_256x64x20": small,
     }
     # compulsory bytes of the whole step (inputs read once, species read and
     # written once): 4*(5*N + 2*N_u + 2*N_v) + S*2*4*N
