@@ -70,6 +70,12 @@ def set_bf16(case):
             d.storage[n] = (_lib.STORE_BF16, synthetic.pow2_scale(a, _lib.STORE_BF16))
 
 
+# profiler kernel names of the timed kernels
+KERNEL_SYMBOL = {"fused_corrective_pass": "k_corr", "fused_first_pass": "k_first",
+                 "face_coefficients": "k_coef", "courant": "k_courant", "upwind": "k_upwind",
+                 "pseudo_velocity": "k_pseudo", "fct": "k_fct", "copy": "k_copy"}
+
+
 def read_traffic(kernel: str):
     """HBM bytes per launch of a kernel from the committed PMC summary
     (profiles/traffic_*.txt: lines 'kernel fetch_bytes write_bytes')."""
@@ -83,6 +89,21 @@ def read_traffic(kernel: str):
                 except ValueError:
                     pass
     return None
+
+
+def step_bytes(d) -> float:
+    """Compulsory HBM bytes of one step: the nine input fields read once and
+    every species read and written once at its storage width,
+    4*(5*N + 2*N_u + 2*N_v) + sum_s 2*b_s*N."""
+    from hicar_amd import _lib
+    N = d.nx * d.ny * d.nz
+    Nu = (d.nx + 1) * d.nz * d.ny
+    Nv = d.nx * d.nz * (d.ny + 1)
+    b = 4 * (5 * N + 2 * Nu + 2 * Nv)
+    for n in d.species:
+        st = d.storage.get(n, (_lib.STORE_F32, 0))[0]
+        b += 2 * (4 if st == _lib.STORE_F32 else 2) * N
+    return float(b)
 
 
 def cpu_baseline():
@@ -137,7 +158,8 @@ def main() -> int:
         if name == "batched_bf16":
             set_bf16(case)
         t, stats = measure(case, variants[name], args.steps, args.warmup)
-        results[name] = {"seconds": t, "value": cells * S * args.steps / t, "stats": stats}
+        results[name] = {"seconds": t, "value": cells * S * args.steps / t, "stats": stats,
+                         "b_step": step_bytes(d)}
         log(f"  {name}: {results[name]['value'] / 1e9:.2f} G cell-species/s, "
             f"{1e3 * t / args.steps:.2f} ms/step")
     # launch-bound small domain: eager vs graph replay
@@ -155,16 +177,23 @@ def main() -> int:
     best = max(exact_ok or results, key=lambda k: results[k]["value"])
     r = results[best]
     stats = r["stats"]
+    # compulsory bytes of one step (inputs read once, every species read and
+    # written once at its storage width): 4*(5*N + 2*N_u + 2*N_v) + sum_s 2*b_s*N
+    b_step = r["b_step"]
+    per_unit = b_step / (cells * S)  # bytes per cell-species update
     dom_k = max(stats, key=lambda k: stats[k]["total_ms"])
     ks = stats[dom_k]
     avg_s = ks["total_ms"] / ks["launches"] / 1e3
-    bytes_per_launch = ks["alg_bytes"] / ks["launches"]
-    achieved = bytes_per_launch / avg_s
-    traffic = read_traffic({"fused_corrective_pass": "k_corr", "fct": "k_fct"}.get(dom_k, dom_k))
+    # cell-species one launch of the dominant kernel advances: all species for
+    # the batched kernels and the Courant kernel, one species otherwise
+    units = cells * (S if (best != "plain" or dom_k == "courant") else 1)
+    achieved = per_unit * units / avg_s
+    traffic = read_traffic(KERNEL_SYMBOL.get(dom_k, dom_k))
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                 "kernel": dom_k, "avg_launch_ms": avg_s * 1e3,
-                "alg_bytes_per_launch": bytes_per_launch}
+                "bytes_per_cell_species": per_unit, "cell_species_per_launch": units,
+                "kernel_own_bytes_per_launch": ks["alg_bytes"] / ks["launches"]}
     out = {
         "metric": "MPDATA cell-species updates/s & % HBM roofline, plain FP32 vs batched/mixed",
         "value": r["value"],
@@ -186,20 +215,15 @@ def main() -> int:
                    "fct": True, "variant": best, "dt": case.dt},
         "roofline": roofline,
         "variants": {k: {"value": v["value"], "ms_per_step": 1e3 * v["seconds"] / args.steps,
+                         "compulsory_hbm_fraction": v["b_step"] * args.steps / v["seconds"] / HBM_PEAK,
                          "kernels": {kk: {"avg_ms": s["total_ms"] / s["launches"],
                                           "launches": s["launches"],
                                           "achieved_GBps": s["alg_bytes"] / s["total_ms"] / 1e6}
                                      for kk, s in v["stats"].items()}}
                      for k, v in results.items()},
-        "compulsory_hbm_fraction": None,
+        "compulsory_hbm_fraction": b_step * args.steps / r["seconds"] / HBM_PEAK,
         "launch_bound_256x64x20": small,
     }
-    # compulsory bytes of the whole step (inputs read once, species read and
-    # written once): 4*(5*N + 2*N_u + 2*N_v) + S*2*4*N
-    Nu = (d.nx + 1) * d.nz * d.ny
-    Nv = d.nx * d.nz * (d.ny + 1)
-    b_step = 4 * (5 * cells + 2 * Nu + 2 * Nv) + S * 2 * 4 * cells
-    out["compulsory_hbm_fraction"] = b_step * args.steps / r["seconds"] / HBM_PEAK
     if not args.no_cpu:
         log("CPU baseline (oracle, 1 thread) ...")
         out["cpu_baseline"] = cpu_baseline()

@@ -382,6 +382,15 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
         row_in[r] = r < nsr && lane_in && gj >= 0 && gj < ny;
         rowbase[r] = row_in[r] ? (unsigned)gi + sj * (unsigned)gj : 0u;
     }
+    // the owned rows rb, rb+1 with static indices (rb is 0 or 1), so that no
+    // per-lane array is indexed at run time
+    bool own_in[OWN];
+    unsigned own_base[OWN];
+#pragma unroll
+    for (int t = 0; t < OWN; t++) {
+        own_in[t] = rb ? row_in[t + 1] : row_in[t];
+        own_base[t] = rb ? rowbase[t + 1] : rowbase[t];
+    }
 
     // per owned column: z pipeline state
     float z0a[OWN], z0b[OWN], z0c[OWN];          // q0 at levels m-2, m-1, m
@@ -418,12 +427,13 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                 sq0[m % 3][row][lx] = n0[r];
                 sq1[m & 3][row][lx] = n1[r];
             }
-            if (r >= rb && r < rb + OWN) {  // own column: q0 at levels m-2, m-1, m
-                const int t = r - rb;
-                z0a[t] = z0b[t];
-                z0b[t] = z0c[t];
-                z0c[t] = (m < nz) ? n0[r] : 0.0f;
-            }
+        }
+#pragma unroll
+        for (int t = 0; t < OWN; t++) {  // own column: q0 at levels m-2, m-1, m
+            const float nv = rb ? n0[t + 1] : n0[t];
+            z0a[t] = z0b[t];
+            z0b[t] = z0c[t];
+            z0c[t] = (m < nz) ? nv : 0.0f;
         }
         if (m + 1 < nz) {
 #pragma unroll
@@ -441,15 +451,14 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
         auto load_coef = [&]() {
 #pragma unroll
             for (int t = 0; t < OWN; t++) {
-                const int r = rb + t;
 #pragma unroll
                 for (int e = 0; e < 4; e++) kx[t][e] = 0.0f;
 #pragma unroll
                 for (int e = 0; e < 5; e++) kz[t][e] = 0.0f;
                 kg[t][0] = kg[t][1] = 1.0f;
-                if (row_in[r]) {
+                if (own_in[t]) {
                     if (L2 >= 0) {
-                        const unsigned c = rowbase[r] + sk * (unsigned)L2;
+                        const unsigned c = own_base[t] + sk * (unsigned)L2;
                         const float4 v = gx[c];
                         kx[t][0] = v.x;
                         kx[t][1] = v.y;
@@ -459,7 +468,7 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                         kg[t][1] = Gzp[c];
                     }
                     if (L1 >= 0 && L1 <= nz - 2) {
-                        const unsigned c = rowbase[r] + sk * (unsigned)L1;
+                        const unsigned c = own_base[t] + sk * (unsigned)L1;
                         const float4 v = gz[c];
                         kz[t][0] = v.x;
                         kz[t][1] = v.y;
@@ -496,7 +505,7 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                 const int row = ly + 8 * (rb + t);
                 const int gj = j0 + row;
                 float wv = 0.0f, fv = 0.0f;
-                if (kz_ok && lane_out && row_in[rb + t]) {
+                if (kz_ok && lane_out && own_in[t]) {
                     const float(*qa)[CX] = sq1[L1 & 3];  // level k
                     const float(*qb)[CX] = sq1[m & 3];   // level k+1
                     const float l = qa[row][lx], r = qb[row][lx];
@@ -531,7 +540,7 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                 const int row = ly + 8 * (rb + t);
                 const int gj = j0 + row;
                 float uv = 0.0f, fv = 0.0f;
-                if (lx >= 1 && gi >= 1 && row_in[rb + t]) {
+                if (lx >= 1 && gi >= 1 && own_in[t]) {
                     const float r = q1m[row][lx], l = q1m[row][lx - 1];
                     float val = kx[t][0] * (r - l) / (r + l + eps);
                     if (gj >= 1 && gj <= ny - 2) {  // UxV (:145-152)
@@ -698,8 +707,8 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
             for (int t = 0; t < OWN; t++) {
                 const int row = ly + 8 * (rb + t);
                 const int gj = j0 + row;
-                if (lane_out && row_in[rb + t]) {
-                    const unsigned c = rowbase[rb + t] + sk * (unsigned)L2;
+                if (lane_out && own_in[t]) {
+                    const unsigned c = own_base[t] + sk * (unsigned)L2;
                     const float qc1 = q1m[row][lx];
                     float q = qc1;
                     if (!(gi == 0 || gi == nx - 1 || gj == 0 || gj == ny - 1)) {

@@ -51,9 +51,11 @@ typedef struct {
  *                   mpdata_compute_wind (adv_mpdata.f90:500-502) and syncs
  *   rk3             &time_parameters RK3: MPDATA under RK3 does nothing
  *                   (advection_driver.f90:112-115); advect returns 0
- *   kernel_variant  no reference counterpart: 0 = "exact" kernels that keep
- *                   the reference's IEEE operation order (bit-identical to a
- *                   strict FP32 build), 1 = "fast" fused/batched kernels. */
+ *   kernel_variant  no reference counterpart: 0 = one kernel per MPDATA pass
+ *                   and species, 1 = fused kernels batched over all species
+ *                   of the handle (about twice as fast); both keep the reference's
+ *                   IEEE FP32 operation order and give bit-identical results
+ *                   for FP32 storage. */
 typedef struct {
     int mpdata_order;
     int mpdata_fct;

@@ -16,8 +16,11 @@ dropped term, a wrong sign or index or a transposed operand fails one of them:
   fct (a7)            the FCT bound property (result inside the local
                       min/max of q0 and q1) incl. the w-column ends; the
                       step-function scenario of test_mpdata.f90:51-111
-  advect3d (a4)       order 1 == upwind; x/y mirror and x<->y transposition
-                      symmetry of the full order-2/3 FCT step
+  advect3d (a4)       order 1 == upwind; 1-D order 2 == textbook MPDATA
+                      with the 0.5 damping and the dz rescaling of w (FP64
+                      NumPy restatement); order 3 == order 2 plus one pass
+                      with q0 = q1 (the iord >= 3 quirk); x/y mirror and
+                      x<->y transposition symmetry of the full FCT step
   advect (a1)         RK3 no-op; species independence; density shadowing
                       (Courant numbers with rho=1, divergence with density)
   divergence, mass    closed forms
@@ -438,6 +441,97 @@ def test_order1_equals_upwind(orc):
     b = orc.upwind(q, U, V, W, rho, dz, jaco)
     np.testing.assert_array_equal(a, b)
     np.testing.assert_array_equal(orc.advect3d(q, rho, jaco, dz, U, V, W, 0, True), q)
+
+
+def _donor_1d(q, c, top=None):
+    """Textbook donor cell along the last axis with Courant numbers c at the
+    faces (c[..., m] is the face below cell m, m >= 1; for the vertical,
+    face m is the top of cell m-1).  Without ``top`` the two end cells are
+    not updated (the lateral ring); with ``top`` the first cell has a closed
+    bottom and the last cell loses q_last * top through the lid."""
+    F = np.zeros(q.shape[:-1] + (q.shape[-1] + 1,))
+    F[..., 1:-1] = np.maximum(c[..., 1:], 0) * q[..., :-1] + np.minimum(c[..., 1:], 0) * q[..., 1:]
+    out = q.copy()
+    if top is None:
+        out[..., 1:-1] = q[..., 1:-1] - (F[..., 2:-1] - F[..., 1:-2])
+    else:
+        F[..., -1] = q[..., -1] * top
+        out = q - (F[..., 1:] - F[..., :-1])
+    return out
+
+
+def _antidiff_1d(q, c):
+    """Smolarkiewicz (1983) antidiffusive Courant number at every interior
+    face, (|c| - c^2) (r - l) / (r + l + 1e-10)."""
+    a = np.zeros_like(c)
+    r, l = q[..., 1:], q[..., :-1]
+    a[..., 1:] = (np.abs(c[..., 1:]) - c[..., 1:] ** 2) * (r - l) / (r + l + 1e-10)
+    return a
+
+
+@pytest.mark.parametrize("axis", ["x", "z"])
+def test_order2_1d_is_textbook_mpdata_halved(orc, axis):
+    """G = 1, one wind component: order 2 without FCT is two textbook
+    donor-cell passes, the second with the antidiffusive Courant number times
+    0.5 (:378-380).  In z the reference's W is a Courant number times dz and
+    the pseudo-velocity is formed from W/dz and scaled back by dz (:374,380),
+    so a uniform dz = 2.5 must drop out; the lid flux of pass 1 is
+    q(top)*W(top) and that of pass 2 is zero (:96-97, :211-212).  The
+    expectation is an independent FP64 NumPy restatement."""
+    nx, ny, nz = 12, 4, 9
+    shape = (ny, nz, nx)
+    r = rnd(21)
+    q = orc.arr(r.random(shape) + 0.2)
+    U, V, W = (zeros(orc, shape) for _ in range(3))
+    dzv = 2.5
+    G, dz, rho = fields(orc, shape, dz=dzv)
+    q64 = q.astype(np.float64)
+    if axis == "x":
+        U[:, :, 1:] = r.uniform(-0.8, 0.8, (ny, nz, nx - 1))
+        c = U.astype(np.float64)
+        q1 = _donor_1d(q64, c)
+        q1[0], q1[-1] = q64[0], q64[-1]  # rows j = jms, jme are the ring
+        exp = _donor_1d(q1, 0.5 * _antidiff_1d(q1, c))
+        exp[0], exp[-1] = q64[0], q64[-1]
+    else:
+        W[...] = r.uniform(-0.8, 0.8, shape) * dzv
+        cw = W.astype(np.float64) / dzv  # Courant number of the top face of each level
+        qt = np.transpose(q64, (0, 2, 1))  # (ny, nx, nz): levels last
+        c = np.zeros(qt.shape)
+        c[..., 1:] = np.transpose(cw, (0, 2, 1))[..., :-1]
+        q1 = _donor_1d(qt, c, top=np.transpose(cw, (0, 2, 1))[..., -1])
+        exp = _donor_1d(q1, 0.5 * _antidiff_1d(q1, c), top=np.zeros(qt.shape[:-1]))
+        exp = np.transpose(exp, (0, 2, 1))
+        exp[0], exp[-1], exp[:, :, 0], exp[:, :, -1] = q64[0], q64[-1], q64[:, :, 0], q64[:, :, -1]
+    out = orc.advect3d(q, rho, G, dz, U, V, W, 2, False).astype(np.float64)
+    tol = 1e-12 if orc.dtype == np.float64 else 3e-6
+    np.testing.assert_allclose(out, exp, rtol=0, atol=tol * np.abs(exp).max())
+    # without the 0.5 damping the answer moves well beyond the tolerance
+    if axis == "x":
+        undamped = _donor_1d(q1, _antidiff_1d(q1, c))
+        undamped[0], undamped[-1] = q64[0], q64[-1]
+        assert np.abs(undamped - exp).max() > 1e3 * tol * np.abs(exp).max()
+
+
+@pytest.mark.parametrize("fct", [True, False])
+def test_order3_is_order2_then_one_more_corrective_pass(orc, fct):
+    """The iord >= 3 quirk (:374,382,388-397): the third pass forms its
+    pseudo-velocities from the physical Courant numbers, and its FCT bounds
+    take q0 = q1 = the order-2 result.  Composed from the separately pinned
+    functions; using the original field for q0 or the previous pass's
+    pseudo-velocities fails it."""
+    q, U, V, W, rho, jaco, dz = random_case(orc, 10, 8, 6, 22)
+    r2 = orc.advect3d(q, rho, jaco, dz, U, V, W, 2, fct)
+    u2, v2, w2 = scaled_fluxes(orc, r2, U, V, W, jaco * rho, dz)
+    if fct:
+        u2, v2, w2 = orc.flux_limiter(r2, r2, u2, v2, w2)
+    exp = orc.upwind(r2, u2, v2, w2, rho, dz, jaco)
+    np.testing.assert_array_equal(orc.advect3d(q, rho, jaco, dz, U, V, W, 3, fct), exp)
+    if fct:  # q0 = the original field gives a different answer
+        u2b, v2b, w2b = scaled_fluxes(orc, r2, U, V, W, jaco * rho, dz)
+        u2b, v2b, w2b = orc.flux_limiter(q, r2, u2b, v2b, w2b)
+        wrong = orc.upwind(r2, u2b, v2b, w2b, rho, dz, jaco)
+        assert np.abs(wrong - exp).max() > 0
 
 
 @pytest.mark.parametrize("order,fct", [(2, True), (3, True), (2, False)])
