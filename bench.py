@@ -32,6 +32,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+# wave64 VALU instructions per second: 256 CUs x 4 SIMDs x 2.4 GHz, 4 cycles each
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 
 
 def log(msg: str) -> None:
@@ -104,6 +106,18 @@ def step_bytes(d) -> float:
         st = d.storage.get(n, (_lib.STORE_F32, 0))[0]
         b += 2 * (4 if st == _lib.STORE_F32 else 2) * N
     return float(b)
+
+
+def read_valu(kernel: str):
+    """VALU wave-instructions per launch of a kernel from the committed PMC
+    summary (lines 'valu kernel count')."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "traffic_*.txt"))):
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "valu" and parts[1] == kernel:
+                return float(parts[2])
+    return None
 
 
 def cpu_baseline():
@@ -194,6 +208,13 @@ def main() -> int:
                 "kernel": dom_k, "avg_launch_ms": avg_s * 1e3,
                 "bytes_per_cell_species": per_unit, "cell_species_per_launch": units,
                 "kernel_own_bytes_per_launch": ks["alg_bytes"] / ks["launches"]}
+    # the kernel's other roof: VALU issue (instruction count per launch from
+    # the committed SQ_INSTS_VALU profile, over the live launch duration)
+    vi = read_valu(KERNEL_SYMBOL.get(dom_k, dom_k))
+    if vi is not None and best != "plain":
+        roofline["valu"] = {"wave_instr_per_launch": vi, "achieved": vi / avg_s / 1e9,
+                            "peak": VALU_ISSUE_PEAK / 1e9, "unit": "G wave-instr/s",
+                            "frac": vi / avg_s / VALU_ISSUE_PEAK}
     out = {
         "metric": "MPDATA cell-species updates/s & % HBM roofline, plain FP32 vs batched/mixed",
         "value": r["value"],
