@@ -2,7 +2,8 @@
 //
 // One advect() step of mpdata_order n (adv_mpdata.f90:352-413) becomes
 //   k_coef     species-invariant face coefficients of the pseudo-velocities
-//              and the divergence reciprocals, once per step
+//              (|U|(1-|U|/avg G), the cross-term wind products and the face
+//              sums of G), once per step
 //   k_first    the donor-cell pass (upwind_advection, :43-103) for every
 //              species in one launch, the Courant numbers loaded once per
 //              cell and reused across species (bit-identical to the
