@@ -36,6 +36,7 @@ EXPORTS = (
     "icar_adv_kernel_stats", "icar_adv_set_graph", "icar_adv_ensemble_create",
     "icar_adv_ensemble_destroy", "icar_adv_ensemble_run", "icar_adv_ensemble_sync",
     "icar_adv_ensemble_last_error", "icar_adv_ensemble_profile", "icar_adv_ensemble_kernel_stats",
+    "icar_adv_launch_config", "icar_adv_ensemble_launch_config",
 )
 
 
@@ -99,6 +100,8 @@ def load() -> ctypes.CDLL:
         "icar_adv_ensemble_last_error": ([vp], ctypes.c_char_p),
         "icar_adv_ensemble_profile": ([vp, ip], ip),
         "icar_adv_ensemble_kernel_stats": ([vp, ip, dp, ctypes.POINTER(ctypes.c_longlong), dp], ip),
+        "icar_adv_launch_config": ([vp, i3, i3], ip),
+        "icar_adv_ensemble_launch_config": ([vp, i3, i3], ip),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -113,6 +113,12 @@ class MpdataAdvection:
     def profile(self, enable: bool) -> None:
         self._check(self.lib.icar_adv_profile(self.h, int(enable)))
 
+    def launch_config(self) -> tuple:
+        """(corr_cfg, ksplit) the fused path chose on its first step."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.icar_adv_launch_config(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def kernel_stats(self) -> dict:
         out = {}
         for kid in range(self.lib.icar_adv_kernel_count()):
@@ -154,6 +160,12 @@ class Ensemble:
 
     def profile(self, enable: bool) -> None:
         self._check(self.lib.icar_adv_ensemble_profile(self.e, int(enable)))
+
+    def launch_config(self) -> tuple:
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.icar_adv_ensemble_launch_config(self.e, ctypes.byref(a),
+                                                             ctypes.byref(b)))
+        return a.value, b.value
 
     def kernel_stats(self) -> dict:
         out = {}

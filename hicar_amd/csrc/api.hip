@@ -721,7 +721,7 @@ int icar_adv_run(void *handle, float dt, int nsteps, int bc_x, int bc_y)
         rc = ensure_batch(h);
         if (rc) return rc;
         std::string e;
-        rc = fast_batch_prepare(h->batch, &dt, &e);
+        rc = fast_batch_ready(h->batch, h->stream, &dt, &e);  // autotuning, if due, runs here
         if (rc) return set_err(h, rc, "%s", e.c_str());
     }
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
@@ -757,6 +757,15 @@ int icar_adv_set_graph(void *handle, int enable)
     int rc = check_handle(handle, &h);
     if (rc) return rc;
     h->use_graph = enable != 0;
+    return 0;
+}
+
+int icar_adv_launch_config(void *handle, int *corr_cfg, int *ksplit)
+{
+    Handle *h;
+    int rc = check_handle(handle, &h);
+    if (rc) return rc;
+    fast_batch_config(h->batch, corr_cfg, ksplit);
     return 0;
 }
 
@@ -950,7 +959,7 @@ int icar_adv_ensemble_run(void *ens, const float *dt, int nsteps, int bc_x, int 
         e->gexec = nullptr;
     }
     std::string m;
-    int rc = fast_batch_prepare(e->batch, dt, &m);
+    int rc = fast_batch_ready(e->batch, e->stream, dt, &m);
     if (rc) return ens_err(e, rc, "%s", m.c_str());
     hipError_t x = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
     if (x != hipSuccess) return ens_err(e, (int)x, "hipStreamBeginCapture: %s", hipGetErrorString(x));
@@ -1002,6 +1011,14 @@ int icar_adv_ensemble_profile(void *ens, int enable)
     if (enable)
         for (int k = 0; k < K_COUNT; k++) e->acc_ms[k] = e->acc_n[k] = e->acc_bytes[k] = 0;
     e->prof = enable != 0;
+    return 0;
+}
+
+int icar_adv_ensemble_launch_config(void *ens, int *corr_cfg, int *ksplit)
+{
+    Ensemble *e = static_cast<Ensemble *>(ens);
+    if (!e) return ens_err(nullptr, ICAR_ADV_EINVAL, "NULL ensemble");
+    fast_batch_config(e->batch, corr_cfg, ksplit);
     return 0;
 }
 

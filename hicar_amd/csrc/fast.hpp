@@ -44,6 +44,17 @@ void fast_batch_destroy(FastBatch *b);
 // changed; fast_batch_advect calls it, graph capture calls it beforehand.
 int fast_batch_prepare(FastBatch *b, const float *dt, std::string *err);
 
+// Chooses the corrective kernel's launch configuration and column chunking
+// for this batch by timing the candidates once (the result is cached by
+// batch shape for the process); a no-op once chosen or while the stream is
+// being captured.  fast_batch_ready = prepare + tune; call it before a graph
+// capture.
+int fast_batch_tune(FastBatch *b, hipStream_t s, std::string *err);
+int fast_batch_ready(FastBatch *b, hipStream_t s, const float *dt, std::string *err);
+// The chosen configuration (corr_cfg 0..3) and the column chunks of the
+// first domain.
+void fast_batch_config(const FastBatch *b, int *corr_cfg, int *ksplit);
+
 using TimedFn = std::function<void(int id, double bytes, const std::function<void()> &)>;
 int fast_batch_advect(FastBatch *b, hipStream_t s, const float *dt, const TimedFn &timed,
                       std::string *err);

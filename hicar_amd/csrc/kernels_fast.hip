@@ -33,6 +33,8 @@
 #include <hip/hip_fp16.h>
 
 #include <cstdlib>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -96,6 +98,7 @@ struct DomDesc {
     unsigned cb0, bx;     // first global cell block; cell blocks along x
     unsigned w0;          // first global corrective work item
     int tiles_x, nspec;
+    int nks, kch;         // corrective kernel: column chunks along k and levels per chunk
     float dx, dt;
     int advect_density;
     DomainFields f;       // raw inputs
@@ -346,9 +349,19 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
     const DomDesc &D = DD[find_dom_w(DD, nd, w)];
     const unsigned lw = w - D.w0;
     const int s = (int)(lw % (unsigned)D.nspec);
-    const int tile = (int)(lw / (unsigned)D.nspec);
+    const unsigned tk = lw / (unsigned)D.nspec;
+    const int kc = (int)(tk % (unsigned)D.nks);  // column chunk
+    const int tile = (int)(tk / (unsigned)D.nks);
     const int tx = tile % D.tiles_x, ty = tile / D.tiles_x;
     const int nx = D.d.nx, ny = D.d.ny, nz = D.d.nz;
+    // Levels [k0, k1) of the column are written by this block.  The march
+    // starts three levels below k0 so that the vertical pipeline (q0 of
+    // three levels, the two z faces below, the betas and flux of the level
+    // below) holds the values a march from the ground would hold when it
+    // reaches k0; the warm-up levels are computed but not stored.
+    const int k0 = kc * D.kch;
+    const int k1 = min(nz, k0 + D.kch);
+    const int m0 = max(0, k0 - 3);
     // 32-bit element offsets (the host checks nx*ny*nz < 2^32) so that the
     // loads use a scalar base and one VGPR offset
     const unsigned sk = (unsigned)nx, sj = (unsigned)nx * (unsigned)nz;
@@ -412,12 +425,13 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
     for (int r = 0; r < SR; r++) {
         n0[r] = n1[r] = 0.0f;
         if (row_in[r]) {
-            n0[r] = ld_q(p.q0, rowbase[r], p.kind0, p.scale0);
-            n1[r] = same ? n0[r] : p.q1[rowbase[r]];
+            const unsigned c = rowbase[r] + sk * (unsigned)m0;
+            n0[r] = ld_q(p.q0, c, p.kind0, p.scale0);
+            n1[r] = same ? n0[r] : p.q1[c];
         }
     }
 
-    for (int m = 0; m <= nz + 1; m++) {
+    for (int m = m0; m <= k1 + 1; m++) {
         const int L1 = m - 1, L2 = m - 2;
         // ---------------- (A) stage level m, prefetch level m+1 and the
         //                  coefficients of this iteration
@@ -708,7 +722,7 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
             for (int t = 0; t < OWN; t++) {
                 const int row = ly + 8 * (rb + t);
                 const int gj = j0 + row;
-                if (lane_out && own_in[t]) {
+                if (lane_out && own_in[t] && L2 >= k0) {
                     const unsigned c = own_base[t] + sk * (unsigned)L2;
                     const float qc1 = q1m[row][lx];
                     float q = qc1;
@@ -792,7 +806,10 @@ struct FastBatch {
     std::vector<float> dts;              // dt values in the device tables
     unsigned total_cb = 0, total_work = 0;
     double cells = 0, cell_species = 0;  // for the algorithmic byte counts
-    int corr_cfg = 1;  // corrective kernel: bit 0 -> 4 waves/SIMD, bit 1 -> no prefetch
+    int corr_cfg = 1;    // corrective kernel: bit 0 -> 4 waves/SIMD, bit 1 -> no prefetch
+    int nks = 1;         // requested column chunks along k
+    bool tuned = false;  // launch configuration chosen (autotuned or fixed by the environment)
+    bool dirty = true;   // device tables out of date
 };
 
 namespace {
@@ -929,6 +946,175 @@ void fast_set_parity(FastState *st, int parity)
 
 const char *fast_error(FastState *st) { return st ? st->err.c_str() : "no fused state"; }
 
+namespace {
+
+constexpr int MIN_CHUNK = 4;  // levels per column chunk at least
+
+// Column chunks of every domain for about nks chunks per column, and the
+// work-item layout (tile, chunk, species) that follows from them.
+void set_ksplit(FastBatch *b, int nks)
+{
+    b->nks = nks < 1 ? 1 : nks;
+    unsigned w = 0;
+    for (size_t k = 0; k < b->doms.size(); k++) {
+        const Dims &d = b->doms[k].st->d;
+        int kch = (d.nz + b->nks - 1) / b->nks;
+        if (kch < MIN_CHUNK) kch = MIN_CHUNK;
+        if (kch > d.nz) kch = d.nz;
+        const int chunks = (d.nz + kch - 1) / kch;
+        const int tiles = ((d.nx + CTI - 1) / CTI) * ((d.ny + CTJ - 1) / CTJ);
+        for (int par = 0; par < 2; par++) {
+            DomDesc &E = b->h_desc[par][k];
+            E.w0 = w;
+            E.nks = chunks;
+            E.kch = kch;
+        }
+        w += (unsigned)(tiles * chunks * (int)b->doms[k].st->sp.size());
+    }
+    b->total_work = w;
+    b->dirty = true;
+}
+
+int upload_tables(FastBatch *b, std::string *err)
+{
+    const size_t n = b->doms.size();
+    for (int par = 0; par < 2; par++)
+        if (hipMemcpy(b->d_desc + (size_t)par * n, b->h_desc[par].data(), n * sizeof(DomDesc),
+                      hipMemcpyHostToDevice) != hipSuccess) {
+            *err = "upload of the domain table failed";
+            return ICAR_ADV_ENOMEM;
+        }
+    b->dirty = false;
+    return 0;
+}
+
+void launch_corr(const FastBatch *b, hipStream_t s, const DomDesc *D, int n, int pass, int cfg)
+{
+    const unsigned nw = b->total_work;
+    const unsigned nb = (nw + 7) / 8 * 8;
+    switch ((b->fct ? 4 : 0) + cfg) {
+    case 0: hipLaunchKernelGGL((k_corr<false, 2, true>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+    case 1: hipLaunchKernelGGL((k_corr<false, 4, true>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+    case 2: hipLaunchKernelGGL((k_corr<false, 2, false>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+    case 3: hipLaunchKernelGGL((k_corr<false, 4, false>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+    case 4: hipLaunchKernelGGL((k_corr<true, 2, true>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+    case 5: hipLaunchKernelGGL((k_corr<true, 4, true>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+    case 6: hipLaunchKernelGGL((k_corr<true, 2, false>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+    default: hipLaunchKernelGGL((k_corr<true, 4, false>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+    }
+}
+
+// Tuned configurations by batch shape, shared by every batch of the process.
+std::mutex g_tune_mu;
+std::map<std::string, std::pair<int, int>> g_tuned;
+
+std::string tune_key(const FastBatch *b)
+{
+    std::string k = std::to_string(b->order) + "/" + std::to_string(b->fct);
+    for (const auto &dm : b->doms)
+        k += ":" + std::to_string(dm.st->d.nx) + "x" + std::to_string(dm.st->d.ny) + "x" +
+             std::to_string(dm.st->d.nz) + "s" + std::to_string(dm.st->sp.size());
+    return k;
+}
+
+}  // namespace
+
+int fast_batch_tune(FastBatch *b, hipStream_t s, std::string *err)
+{
+    if (b->tuned) return 0;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
+        return 0;  // never time inside a graph capture: keep the defaults
+    const bool fix_cfg = getenv("ICAR_ADV_CORR_CFG") != nullptr;
+    const bool fix_ks = getenv("ICAR_ADV_KSPLIT") != nullptr;
+    const bool cached = !fix_cfg && !fix_ks;  // results with a fixed part are not shared
+    const std::string key = tune_key(b);
+    if (cached) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            b->corr_cfg = it->second.first;
+            set_ksplit(b, it->second.second);
+            b->tuned = true;
+            return 0;
+        }
+    }
+    // candidates: the launch configurations, and column chunks while one
+    // chunk per column leaves the GPU with few blocks
+    std::vector<int> cfgs = fix_cfg ? std::vector<int>{b->corr_cfg} : std::vector<int>{1, 0, 3, 2};
+    std::vector<int> kss{b->nks};
+    if (!fix_ks) {
+        set_ksplit(b, 1);
+        const unsigned base = b->total_work;
+        kss = {1};
+        if (base < 4096)
+            for (int k : {2, 3, 4, 6, 8})
+                if ((unsigned)k * base <= 16384) kss.push_back(k);
+    }
+    const int n = (int)b->doms.size();
+    const int par = b->doms[0].st->cur;
+    // The timed launches are pass 2, which writes only the FP32 pass buffer
+    // (order >= 3) or the alternate home buffer (order 2): both are written
+    // again by the step that follows before anything reads them.
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return 0;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return 0;
+    }
+    float best = 1e30f;
+    int best_cfg = b->corr_cfg, best_ks = kss[0];
+    int rc = 0;
+    std::vector<std::vector<int>> seen;  // chunk counts already timed
+    for (int ks : kss) {
+        set_ksplit(b, ks);
+        std::vector<int> sig;
+        for (int k = 0; k < n; k++) sig.push_back(b->h_desc[0][k].nks);
+        bool dup = false;
+        for (const auto &v : seen) dup = dup || v == sig;
+        if (dup) continue;
+        seen.push_back(sig);
+        if ((rc = upload_tables(b, err))) break;
+        const DomDesc *D = b->d_desc + (size_t)par * n;
+        for (int cfg : cfgs) {
+            launch_corr(b, s, D, n, 2, cfg);  // warm-up
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < 2; r++) launch_corr(b, s, D, n, 2, cfg);
+            (void)hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) {
+                *err = "autotuning launch failed";
+                rc = ICAR_ADV_EINVAL;
+                break;
+            }
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best * 0.97f) {  // prefer the earlier (simpler) candidate on near-ties
+                best = ms;
+                best_cfg = cfg;
+                best_ks = ks;
+            }
+        }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    b->corr_cfg = best_cfg;
+    set_ksplit(b, best_ks);
+    b->tuned = true;
+    if (cached) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[key] = {best_cfg, best_ks};
+    }
+    return 0;
+}
+
+void fast_batch_config(const FastBatch *b, int *corr_cfg, int *ksplit)
+{
+    if (corr_cfg) *corr_cfg = b ? b->corr_cfg : -1;
+    if (ksplit) *ksplit = b ? b->h_desc[0][0].nks : -1;
+}
+
 FastBatch *fast_batch_create(const FastDomain *doms, int n, int order, int fct, std::string *err)
 {
     if (n < 1 || order < 1 || order > MAXP) {
@@ -939,8 +1125,15 @@ FastBatch *fast_batch_create(const FastDomain *doms, int n, int order, int fct, 
     b->doms.assign(doms, doms + n);
     b->order = order;
     b->fct = fct;
-    if (const char *e = getenv("ICAR_ADV_CORR_CFG")) b->corr_cfg = atoi(e) & 3;
-    unsigned cb = 0, w = 0;
+    // launch configuration: ICAR_ADV_CORR_CFG (0..3) and ICAR_ADV_KSPLIT fix
+    // it, ICAR_ADV_AUTOTUNE=0 keeps the defaults; otherwise the first step
+    // times the candidates on this batch
+    const char *ec = getenv("ICAR_ADV_CORR_CFG"), *ek = getenv("ICAR_ADV_KSPLIT");
+    const char *ea = getenv("ICAR_ADV_AUTOTUNE");
+    if (ec) b->corr_cfg = atoi(ec) & 3;
+    if (ek) b->nks = atoi(ek) < 1 ? 1 : atoi(ek);
+    b->tuned = (ec && ek) || (ea && atoi(ea) == 0) || order < 2;
+    unsigned cb = 0;
     for (int i = 0; i < 2; i++) b->h_desc[i].resize(n);
     for (int k = 0; k < n; k++) {
         FastState *st = doms[k].st;
@@ -966,7 +1159,6 @@ FastBatch *fast_batch_create(const FastDomain *doms, int n, int order, int fct, 
         D.d = d;
         D.cb0 = cb;
         D.bx = (unsigned)((d.nx + 63) / 64);
-        D.w0 = w;
         D.tiles_x = (d.nx + CTI - 1) / CTI;
         D.nspec = S;
         D.dx = doms[k].dx;
@@ -975,9 +1167,7 @@ FastBatch *fast_batch_create(const FastDomain *doms, int n, int order, int fct, 
         D.f = doms[k].f;
         D.sf = doms[k].sf;
         D.coef = st->coef;
-        const int tiles_y = (d.ny + CTJ - 1) / CTJ;
         cb += cell_blocks(d);
-        w += (unsigned)(D.tiles_x * tiles_y * S);
         b->cells += (double)st->N;
         b->cell_species += (double)st->N * S;
         for (int par = 0; par < 2; par++) {
@@ -988,7 +1178,7 @@ FastBatch *fast_batch_create(const FastDomain *doms, int n, int order, int fct, 
         }
     }
     b->total_cb = cb;
-    b->total_work = w;
+    set_ksplit(b, b->nks);
     if (hipMalloc(&b->d_desc, 2 * (size_t)n * sizeof(DomDesc)) != hipSuccess) {
         *err = "hipMalloc of the domain table failed";
         delete b;
@@ -1014,23 +1204,25 @@ int fast_batch_prepare(FastBatch *b, const float *dt, std::string *err)
         }
     bool same = (int)b->dts.size() == n;
     for (int k = 0; same && k < n; k++) same = b->dts[k] == dt[k];
-    if (same) return 0;
+    if (same && !b->dirty) return 0;
     b->dts.assign(dt, dt + n);
     for (int par = 0; par < 2; par++)
         for (int k = 0; k < n; k++) b->h_desc[par][k].dt = dt[k];
-    for (int par = 0; par < 2; par++)
-        if (hipMemcpy(b->d_desc + (size_t)par * n, b->h_desc[par].data(), n * sizeof(DomDesc),
-                      hipMemcpyHostToDevice) != hipSuccess) {
-            *err = "upload of the domain table failed";
-            return ICAR_ADV_ENOMEM;
-        }
-    return 0;
+    return upload_tables(b, err);
+}
+
+int fast_batch_ready(FastBatch *b, hipStream_t s, const float *dt, std::string *err)
+{
+    int rc = fast_batch_prepare(b, dt, err);
+    if (rc || b->tuned) return rc;
+    if ((rc = fast_batch_tune(b, s, err))) return rc;
+    return fast_batch_prepare(b, dt, err);
 }
 
 int fast_batch_advect(FastBatch *b, hipStream_t s, const float *dt, const TimedFn &timed,
                       std::string *err)
 {
-    int rc = fast_batch_prepare(b, dt, err);
+    int rc = fast_batch_ready(b, s, dt, err);
     if (rc) return rc;
     const int n = (int)b->doms.size();
     const int par = b->doms[0].st->cur;
@@ -1050,24 +1242,11 @@ int fast_batch_advect(FastBatch *b, hipStream_t s, const float *dt, const TimedF
     // pass 1: donor cell for every species
     timed(K_FUSED_FIRST, 5.0 * 4 * cells + 2.0 * 4 * cs,
           [&] { hipLaunchKernelGGL(k_first, dim3(b->total_cb), dim3(FBX, FBY), 0, s, D, n); });
-    const unsigned nb = (b->total_work + 7) / 8 * 8;
     for (int pass = 2; pass <= b->order; pass++) {
         // species read (q0 and q1 at pass 2) and written once; coefficients,
         // G, Gz and dz once
         const double bytes = (pass == 2 ? 3.0 : 2.0) * 4 * cs + (NCOEF + 3.0) * 4 * cells;
-        timed(K_FUSED_CORR, bytes, [&] {
-            const unsigned nw = b->total_work;
-            switch ((b->fct ? 4 : 0) + b->corr_cfg) {
-            case 0: hipLaunchKernelGGL((k_corr<false, 2, true>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-            case 1: hipLaunchKernelGGL((k_corr<false, 4, true>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-            case 2: hipLaunchKernelGGL((k_corr<false, 2, false>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-            case 3: hipLaunchKernelGGL((k_corr<false, 4, false>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-            case 4: hipLaunchKernelGGL((k_corr<true, 2, true>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-            case 5: hipLaunchKernelGGL((k_corr<true, 4, true>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-            case 6: hipLaunchKernelGGL((k_corr<true, 2, false>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-            default: hipLaunchKernelGGL((k_corr<true, 4, false>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-            }
-        });
+        timed(K_FUSED_CORR, bytes, [&] { launch_corr(b, s, D, n, pass, b->corr_cfg); });
     }
     if (b->order <= 2)  // the results landed in the other home buffers
         for (auto &dm : b->doms) dm.st->cur = 1 - dm.st->cur;

@@ -170,6 +170,16 @@ const char *icar_adv_kernel_name(int id);
 int icar_adv_kernel_stats(void *handle, int id, double *total_ms, long long *launches,
                           double *alg_bytes);
 
+/* Launch configuration of the fused corrective kernel chosen for this handle
+ * (kernel_variant 1), -1 before the first advect.  The first fused step times
+ * the candidates on the handle's own grid and species count and keeps the
+ * fastest (cached for the process by shape): corr_cfg 0..3 selects the
+ * register budget (2 or 4 waves per SIMD) and whether the next level is
+ * prefetched; ksplit is the number of chunks each column march is split into
+ * (more blocks for small grids).  The environment variables ICAR_ADV_CORR_CFG
+ * and ICAR_ADV_KSPLIT fix them, ICAR_ADV_AUTOTUNE=0 keeps 1 and 1. */
+int icar_adv_launch_config(void *handle, int *corr_cfg, int *ksplit);
+
 /* Records the nsteps loop of icar_adv_run in a HIP graph on its first call
  * and replays it on later calls with the same dt, nsteps and boundary
  * conditions (not while profiling or with debug set). */
@@ -192,6 +202,8 @@ const char *icar_adv_ensemble_last_error(void *ens);
 int icar_adv_ensemble_profile(void *ens, int enable);
 int icar_adv_ensemble_kernel_stats(void *ens, int id, double *total_ms, long long *launches,
                                    double *alg_bytes);
+/* As icar_adv_launch_config, for the ensemble's batch (ksplit of member 0). */
+int icar_adv_ensemble_launch_config(void *ens, int *corr_cfg, int *ksplit);
 
 #ifdef __cplusplus
 }

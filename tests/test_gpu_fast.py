@@ -111,3 +111,68 @@ def test_batched_closed_box_mass():
         st.close()
     rel = np.abs(m1 - m0) / np.abs(m0)
     assert rel.max() < 2e-6, rel
+
+
+@pytest.fixture
+def launch_env(monkeypatch):
+    """Fixes the corrective kernel's launch configuration through the
+    environment the library reads when a handle builds its batch."""
+    def set_cfg(cfg=None, ksplit=None):
+        for k, v in (("ICAR_ADV_CORR_CFG", cfg), ("ICAR_ADV_KSPLIT", ksplit)):
+            if v is None:
+                monkeypatch.delenv(k, raising=False)
+            else:
+                monkeypatch.setenv(k, str(v))
+    return set_cfg
+
+
+@pytest.mark.parametrize("ksplit,order,fct", [(2, 3, True), (3, 2, True), (5, 3, True),
+                                              (4, 2, False)])
+def test_column_chunks_match_oracle(oracle32, launch_env, ksplit, order, fct):
+    """The column march split into chunks (each started three levels early
+    and storing only its own levels) gives the oracle's bits; nz = 17 makes
+    the last chunk short."""
+    launch_env(cfg=1, ksplit=ksplit)
+    case = random_domain(131, 37, 17, 300 + ksplit, order, fct, True, nspec=2)
+    ref = oracle32.run(case.domain, case.options, case.dt, case.steps, 0, 0, fresh_species(case))
+    case.options.kernel_variant = 1
+    st = MpdataAdvection(case.domain, case.options)
+    try:
+        st.run(case.dt, case.steps, 0, 0)
+        st.download()
+        cfg, ks = st.launch_config()
+    finally:
+        st.close()
+    assert cfg == 1 and ks == min(ksplit, -(-17 // 4))
+    assert_same({n: case.domain.species[n] for n in ref}, ref)
+
+
+@pytest.mark.parametrize("cfg", [0, 2, 3])
+def test_launch_configurations_match_oracle(oracle32, launch_env, cfg):
+    launch_env(cfg=cfg, ksplit=1)
+    case = random_domain(131, 37, 9, 400 + cfg, 3, True, True, nspec=2)
+    ref = oracle32.run(case.domain, case.options, case.dt, case.steps, 0, 0, fresh_species(case))
+    assert_same(gpu_run(case, variant=1), ref)
+
+
+def test_autotuned_small_domain_matches_oracle(oracle32, launch_env):
+    """The 256x64x20 ridge domain with one species leaves the GPU with 20
+    column tiles: the autotuner times the launch configurations and column
+    chunkings on the first step, and whatever it picks stays bit-exact."""
+    launch_env()
+    case = synthetic.config1(steps=30)
+    case.options.adv_options.MPDATA_FCT = True
+    ref = oracle32.run(case.domain, case.options, case.dt, case.steps, case.bc_x, case.bc_y,
+                       fresh_species(case))
+    case.options.kernel_variant = 1
+    st = MpdataAdvection(case.domain, case.options)
+    try:
+        assert st.launch_config() == (-1, -1)
+        st.run(case.dt, case.steps, case.bc_x, case.bc_y)
+        st.download()
+        cfg, ks = st.launch_config()
+    finally:
+        st.close()
+    assert 0 <= cfg <= 3 and 1 <= ks <= 5
+    print(f"\n[autotune] 256x64x20, 1 species: corr_cfg {cfg}, column chunks {ks}")
+    assert_same({"qv": case.domain.species["qv"]}, ref)
