@@ -11,7 +11,7 @@ HBM.  value = cell-species updates per second = nx*ny*nz*species*steps / t.
 Variants: "plain" (exact per-species kernels, the drop-in FP32 form),
 "batched" (fused species-batched kernels, FP32, bit-identical to plain) and
 "batched_bf16" (the same with the hydrometeor masses and numbers stored in
-BF16 with power-of-two scaling, qv and theta FP32).  value is the best
+BF16 with power-of-two scaling and stochastic rounding, qv and theta FP32).  value is the best
 bit-exact FP32 variant; the others are reported beside it, with a small
 launch-bound case (the 256x64x20 ridge domain) run eagerly and replayed
 from a HIP graph.
@@ -69,7 +69,7 @@ def set_bf16(case):
     d = case.domain
     for n, a in d.species.items():
         if n not in ("qv", "theta"):
-            d.storage[n] = (_lib.STORE_BF16, synthetic.pow2_scale(a, _lib.STORE_BF16))
+            d.storage[n] = (_lib.STORE_BF16_SR, synthetic.pow2_scale(a, _lib.STORE_BF16_SR))
 
 
 # profiler kernel names of the timed kernels

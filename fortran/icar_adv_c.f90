@@ -21,6 +21,7 @@ module icar_adv_c
     public :: ICAR_ADV_FIELD_JACOBIAN, ICAR_ADV_FIELD_JACOBIAN_U, ICAR_ADV_FIELD_JACOBIAN_V
     public :: ICAR_ADV_FIELD_JACOBIAN_W, ICAR_ADV_FIELD_ADVECTION_DZ
     public :: ICAR_ADV_STORE_F32, ICAR_ADV_STORE_BF16, ICAR_ADV_STORE_F16
+    public :: ICAR_ADV_STORE_BF16_SR, ICAR_ADV_STORE_F16_SR
     public :: ICAR_ADV_MASK_ALL_FIELDS, ICAR_ADV_MASK_ALL_SPECIES
 
     integer(c_int), parameter :: ICAR_ADV_FIELD_U = 0, ICAR_ADV_FIELD_V = 1, ICAR_ADV_FIELD_W = 2
@@ -28,6 +29,7 @@ module icar_adv_c
     integer(c_int), parameter :: ICAR_ADV_FIELD_JACOBIAN_U = 5, ICAR_ADV_FIELD_JACOBIAN_V = 6
     integer(c_int), parameter :: ICAR_ADV_FIELD_JACOBIAN_W = 7, ICAR_ADV_FIELD_ADVECTION_DZ = 8
     integer(c_int), parameter :: ICAR_ADV_STORE_F32 = 0, ICAR_ADV_STORE_BF16 = 1, ICAR_ADV_STORE_F16 = 2
+    integer(c_int), parameter :: ICAR_ADV_STORE_BF16_SR = 3, ICAR_ADV_STORE_F16_SR = 4
     integer(c_long_long), parameter :: ICAR_ADV_MASK_ALL_FIELDS = int(z'1FF', c_long_long)
     ! bits 16..63 (species 0..47); written as a negative two's-complement value
     integer(c_long_long), parameter :: ICAR_ADV_MASK_ALL_SPECIES = -65536_c_long_long

@@ -18,7 +18,8 @@ FIELD_JACOBIAN_U, FIELD_JACOBIAN_V, FIELD_JACOBIAN_W, FIELD_ADVECTION_DZ = 5, 6,
 NFIELDS = 9
 FIELD_NAMES = ("u", "v", "w", "density", "jacobian", "jacobian_u", "jacobian_v",
                "jacobian_w", "advection_dz")
-STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
+STORE_F32, STORE_BF16, STORE_F16, STORE_BF16_SR, STORE_F16_SR = 0, 1, 2, 3, 4
+STORE_NAMES = {0: "f32", 1: "bf16", 2: "f16", 3: "bf16-sr", 4: "f16-sr"}
 MAX_SPECIES = 48
 MASK_ALL_FIELDS = 0x1FF
 MASK_ALL_SPECIES = 0xFFFFFFFFFFFF0000

@@ -507,8 +507,7 @@ int icar_adv_bind_species(void *handle, int idx, float *host, int storage, int p
         return set_err(h, ICAR_ADV_ERANGE, "species index %d outside [0,%d)", idx,
                        ICAR_ADV_MAX_SPECIES);
     if (!host) return set_err(h, ICAR_ADV_EINVAL, "NULL array for species %d", idx);
-    if (storage != ICAR_ADV_STORE_F32 && storage != ICAR_ADV_STORE_BF16 &&
-        storage != ICAR_ADV_STORE_F16)
+    if (storage < ICAR_ADV_STORE_F32 || storage > ICAR_ADV_STORE_F16_SR)
         return set_err(h, ICAR_ADV_EINVAL, "unknown storage %d", storage);
     if (storage != ICAR_ADV_STORE_F32 && h->o.kernel_variant == 0)
         return set_err(h, ICAR_ADV_EINVAL,

@@ -48,33 +48,78 @@ namespace {
 // ---------------------------------------------------------------------------
 // species storage: FP32 or 16-bit (BF16 / FP16) holding value * 2^-e
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ bool is_bf16(int kind)
+{
+    return kind == ICAR_ADV_STORE_BF16 || kind == ICAR_ADV_STORE_BF16_SR;
+}
+
 __device__ __forceinline__ float ld_q(const void *p, size_t i, int kind, float scale)
 {
     if (kind == ICAR_ADV_STORE_F32) return static_cast<const float *>(p)[i];
     const uint16_t b = static_cast<const uint16_t *>(p)[i];
-    const float v = (kind == ICAR_ADV_STORE_BF16) ? __uint_as_float((uint32_t)b << 16)
-                                                  : __half2float(__ushort_as_half(b));
+    const float v = is_bf16(kind) ? __uint_as_float((uint32_t)b << 16)
+                                  : __half2float(__ushort_as_half(b));
     return v * scale;
 }
 
-__device__ __forceinline__ uint16_t to16(float v, int kind, float inv_scale)
+// 16 random bits for the stochastic rounding of cell c of species s at step t:
+// a counter-based hash (the murmur3 32-bit finaliser of a mixed key), so the
+// rounding is deterministic and reproducible off the device.
+__device__ __forceinline__ uint32_t sr_bits(uint32_t c, int s, uint32_t t)
+{
+    uint32_t x = (c * 0x9E3779B1u) ^ ((uint32_t)(s + 1) * 0x85EBCA77u) ^ (t * 0xC2B2AE3Du);
+    x ^= x >> 16;
+    x *= 0x85EBCA6Bu;
+    x ^= x >> 13;
+    x *= 0xC2B2AE35u;
+    x ^= x >> 16;
+    return x & 0xFFFFu;
+}
+
+// value -> 16-bit storage of value * inv_scale.  The plain kinds round to
+// nearest even; the _SR kinds round up with probability equal to the
+// discarded fraction of an ulp (rnd: 16 uniform bits), so that increments
+// smaller than half an ulp are not lost on average and the flux form keeps
+// conserving mass.
+__device__ __forceinline__ uint16_t to16(float v, int kind, float inv_scale, uint32_t rnd)
 {
     const float x = v * inv_scale;
-    if (kind == ICAR_ADV_STORE_BF16) {
+    if (is_bf16(kind)) {
         uint32_t u = __float_as_uint(x);
         if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-        u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+        if (kind == ICAR_ADV_STORE_BF16_SR)
+            u += rnd;                         // then truncate: stochastic rounding
+        else
+            u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
         return (uint16_t)(u >> 16);
+    }
+    if (kind == ICAR_ADV_STORE_F16_SR) {
+        const float ax = fabsf(x);
+        if (ax >= 6.103515625e-05f) {  // FP16 normal range: 13 mantissa bits are dropped
+            const uint32_t u = (__float_as_uint(x) + (rnd & 0x1FFFu)) & ~0x1FFFu;
+            return __half_as_ushort(__float2half_rz(__uint_as_float(u)));
+        }
+        // FP16 subnormals are multiples of 2^-24
+        const float a = ax * 16777216.0f;
+        const float k = floorf(a);
+        const float up = ((float)rnd * 1.52587890625e-05f < a - k) ? 1.0f : 0.0f;
+        return __half_as_ushort(__float2half_rn(copysignf((k + up) * 5.9604644775390625e-08f, x)));
     }
     return __half_as_ushort(__float2half_rn(x));
 }
 
-__device__ __forceinline__ void st_q(void *p, size_t i, int kind, float inv_scale, float v)
+__device__ __forceinline__ void st_q(void *p, size_t i, int kind, float inv_scale, float v,
+                                     uint32_t rnd)
 {
     if (kind == ICAR_ADV_STORE_F32)
         static_cast<float *>(p)[i] = v;
     else
-        static_cast<uint16_t *>(p)[i] = to16(v, kind, inv_scale);
+        static_cast<uint16_t *>(p)[i] = to16(v, kind, inv_scale, rnd);
+}
+
+__device__ __forceinline__ bool is_sr(int kind)
+{
+    return kind == ICAR_ADV_STORE_BF16_SR || kind == ICAR_ADV_STORE_F16_SR;
 }
 
 __device__ __forceinline__ float max3(float a, float b, float c) { return fmaxf(fmaxf(a, b), c); }
@@ -104,6 +149,7 @@ struct DomDesc {
     DomainFields f;       // raw inputs
     StepFields sf;        // Courant numbers and products (written by k_courant_b)
     float *coef;          // face coefficients (float4 groups)
+    unsigned *step;       // steps taken (drives the stochastic rounding)
     const SpecPass *pass[MAXP];  // species table of pass p at index p-1
 };
 
@@ -253,6 +299,13 @@ __global__ void __launch_bounds__(256) k_coef(const DomDesc *__restrict__ DD, in
     g[2 * N + c] = make_float4(v[CUZ], v[CZU], v[CZV], v[SGZ]);
 }
 
+// one step taken by every domain of the batch (a vector store per domain)
+__global__ void __launch_bounds__(64) k_tick(const DomDesc *__restrict__ DD, int nd)
+{
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (t < nd) DD[t].step[0] = DD[t].step[0] + 1u;
+}
+
 // ---------------------------------------------------------------------------
 // donor-cell pass for every species (upwind_advection, adv_mpdata.f90:43-103)
 // ---------------------------------------------------------------------------
@@ -290,11 +343,13 @@ __global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ 
         g = G[c];
         gz = Gz[c];
     }
+    const unsigned tstep = *D.step;
     for (int s = 0; s < nspec; s++) {
         const SpecPass p = sp[s];
+        const uint32_t rnd = is_sr(p.kindo) ? sr_bits((uint32_t)c, s, tstep) : 0u;
         const float qc = ld_q(p.q0, c, p.kind0, p.scale0);
         if (ring) {  // :61-63
-            st_q(p.out, c, p.kindo, p.inv_scaleo, qc);
+            st_q(p.out, c, p.kindo, p.inv_scaleo, qc, rnd);
             continue;
         }
         const float qw = ld_q(p.q0, c - 1, p.kind0, p.scale0);
@@ -315,7 +370,7 @@ __global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ 
             const float qd = ld_q(p.q0, c - sk, p.kind0, p.scale0);
             q = q - (flux1(qc, qu, Wt) - flux1(qd, qc, Wb)) / gz;  // :90-91
         }
-        st_q(p.out, c, p.kindo, p.inv_scaleo, q);
+        st_q(p.out, c, p.kindo, p.inv_scaleo, q, rnd);
     }
 }
 
@@ -371,6 +426,7 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
     const int j0 = ty * CTJ - CH;  // global row of staged row 0
     const SpecPass p = D.pass[pass - 1][s];
     const bool same = p.q1 == nullptr;
+    const unsigned tstep = is_sr(p.kindo) ? *D.step : 0u;
     const float eps = 1e-10f, eps15 = 1e-15f;
     const float *__restrict__ Gp = D.sf.G, *__restrict__ Gzp = D.sf.Gz, *__restrict__ dzp = D.f.dz;
     const float4 *__restrict__ gx = reinterpret_cast<const float4 *>(D.coef);
@@ -762,7 +818,8 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                         q = q - ((fr - fl) + (fn - fs)) / kg[t][0];
                         q = q - (Fzc[t] - Fzp[t]) / kg[t][1];
                     }
-                    st_q(p.out, c, p.kindo, p.inv_scaleo, q);
+                    st_q(p.out, c, p.kindo, p.inv_scaleo, q,
+                         is_sr(p.kindo) ? sr_bits(c, s, tstep) : 0u);
                 }
                 Fzp[t] = Fzc[t];
             }
@@ -795,6 +852,7 @@ struct FastState {
     SpecPass *d_tab = nullptr;           // [2 parities][passes][nspec]
     int tab_order = -1;
     size_t tab_cap = 0;
+    unsigned *d_step = nullptr;          // steps taken (stochastic rounding counter)
 };
 
 struct FastBatch {
@@ -810,6 +868,7 @@ struct FastBatch {
     int nks = 1;         // requested column chunks along k
     bool tuned = false;  // launch configuration chosen (autotuned or fixed by the environment)
     bool dirty = true;   // device tables out of date
+    bool has_sr = false; // a species is stored with stochastic rounding
 };
 
 namespace {
@@ -893,7 +952,9 @@ FastState *fast_create(Dims d)
     FastState *st = new FastState();
     st->d = d;
     st->N = (size_t)d.nx * d.ny * d.nz;
-    if (hipMalloc(&st->coef, (size_t)NCOEF * st->N * sizeof(float)) != hipSuccess) {
+    if (hipMalloc(&st->coef, (size_t)NCOEF * st->N * sizeof(float)) != hipSuccess ||
+        hipMalloc(&st->d_step, sizeof(unsigned)) != hipSuccess ||
+        hipMemset(st->d_step, 0, sizeof(unsigned)) != hipSuccess) {
         fast_destroy(st);
         return nullptr;
     }
@@ -906,6 +967,7 @@ void fast_destroy(FastState *st)
     free_species_buffers(st);
     (void)hipFree(st->coef);
     (void)hipFree(st->d_tab);
+    (void)hipFree(st->d_step);
     delete st;
 }
 
@@ -1167,6 +1229,10 @@ FastBatch *fast_batch_create(const FastDomain *doms, int n, int order, int fct, 
         D.f = doms[k].f;
         D.sf = doms[k].sf;
         D.coef = st->coef;
+        D.step = st->d_step;
+        for (const FastSpecies &f : st->sp)
+            b->has_sr = b->has_sr || f.storage == ICAR_ADV_STORE_BF16_SR ||
+                        f.storage == ICAR_ADV_STORE_F16_SR;
         cb += cell_blocks(d);
         b->cells += (double)st->N;
         b->cell_species += (double)st->N * S;
@@ -1248,6 +1314,8 @@ int fast_batch_advect(FastBatch *b, hipStream_t s, const float *dt, const TimedF
         const double bytes = (pass == 2 ? 3.0 : 2.0) * 4 * cs + (NCOEF + 3.0) * 4 * cells;
         timed(K_FUSED_CORR, bytes, [&] { launch_corr(b, s, D, n, pass, b->corr_cfg); });
     }
+    if (b->has_sr)  // next step's rounding draws
+        hipLaunchKernelGGL(k_tick, dim3((n + 63) / 64), dim3(64), 0, s, b->d_desc + (size_t)par * n, n);
     if (b->order <= 2)  // the results landed in the other home buffers
         for (auto &dm : b->doms) dm.st->cur = 1 - dm.st->cur;
     return 0;
@@ -1263,7 +1331,9 @@ __global__ void k_pack(size_t n, const float *__restrict__ src, uint16_t *__rest
 {
     for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
          p += (size_t)gridDim.x * blockDim.x)
-        dst[p] = to16(src[p], storage, inv_scale);
+        dst[p] = to16(src[p], storage == ICAR_ADV_STORE_BF16_SR ? ICAR_ADV_STORE_BF16
+                              : storage == ICAR_ADV_STORE_F16_SR ? ICAR_ADV_STORE_F16 : storage,
+                      inv_scale, 0u);  // uploads round to nearest even
 }
 
 __global__ void k_unpack(size_t n, const uint16_t *__restrict__ src, float *__restrict__ dst,

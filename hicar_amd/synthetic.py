@@ -273,7 +273,7 @@ def pow2_scale(a: np.ndarray, storage: int) -> int:
     if m == 0.0:
         return 0
     e = int(np.ceil(np.log2(m)))
-    return e - 15 if storage == _lib.STORE_F16 else e
+    return e - 15 if storage in (_lib.STORE_F16, _lib.STORE_F16_SR) else e
 
 
 def config4(storage: int = _lib.STORE_BF16, nx=256, ny=64, nz=20, steps=200) -> Case:

@@ -79,8 +79,21 @@ enum {
     ICAR_ADV_NFIELDS = 9
 };
 
-/* Device storage of a species (the host side is always 4-byte real). */
-enum { ICAR_ADV_STORE_F32 = 0, ICAR_ADV_STORE_BF16 = 1, ICAR_ADV_STORE_F16 = 2 };
+/* Device storage of a species (the host side is always 4-byte real).  The
+ * 16-bit kinds hold value * 2^-e and are rounded once per step, after the
+ * FP32 passes: BF16 / F16 to nearest even; BF16_SR / F16_SR stochastically
+ * (round up with probability equal to the discarded fraction of an ulp, the
+ * random bits a counter-based hash of cell, species and step, so runs are
+ * deterministic).  Round-to-nearest loses updates smaller than half an ulp,
+ * which makes the total mass drift (about -1 % in 200 steps in BF16);
+ * stochastic rounding keeps it unbiased.  Uploads round to nearest even. */
+enum {
+    ICAR_ADV_STORE_F32 = 0,
+    ICAR_ADV_STORE_BF16 = 1,
+    ICAR_ADV_STORE_F16 = 2,
+    ICAR_ADV_STORE_BF16_SR = 3,
+    ICAR_ADV_STORE_F16_SR = 4
+};
 
 /* Error codes (negative); positive values are hipError_t. */
 enum {
