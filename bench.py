@@ -59,8 +59,11 @@ def measure(case, variant: int, steps: int, warmup: int, profile: bool = True,
         t1 = time.perf_counter()
         stats = st.kernel_stats() if profile else {}
         st.profile(False)
+        cfg = st.launch_config()
     finally:
         st.close()
+    if profile:
+        stats["_launch_config"] = {"corr_cfg": cfg[0], "column_chunks": cfg[1]}
     return t1 - t0, stats
 
 
@@ -172,8 +175,9 @@ def main() -> int:
         if name == "batched_bf16":
             set_bf16(case)
         t, stats = measure(case, variants[name], args.steps, args.warmup)
+        lc = stats.pop("_launch_config", None)
         results[name] = {"seconds": t, "value": cells * S * args.steps / t, "stats": stats,
-                         "b_step": step_bytes(d)}
+                         "b_step": step_bytes(d), "launch_config": lc}
         log(f"  {name}: {results[name]['value'] / 1e9:.2f} G cell-species/s, "
             f"{1e3 * t / args.steps:.2f} ms/step")
     # launch-bound small domain: eager vs graph replay
@@ -237,6 +241,7 @@ def main() -> int:
         "roofline": roofline,
         "variants": {k: {"value": v["value"], "ms_per_step": 1e3 * v["seconds"] / args.steps,
                          "compulsory_hbm_fraction": v["b_step"] * args.steps / v["seconds"] / HBM_PEAK,
+                         "launch_config": v["launch_config"] if variants[k] == 1 else None,
                          "kernels": {kk: {"avg_ms": s["total_ms"] / s["launches"],
                                           "launches": s["launches"],
                                           "achieved_GBps": s["alg_bytes"] / s["total_ms"] / 1e6}

@@ -391,7 +391,7 @@ constexpr int OWN = 2;            // output rows per wave: ly + 8*(rb + t), t = 
 // WPE: waves per SIMD the register allocation targets (2 blocks of 8 waves
 // per CU need 4); PF: load this iteration's coefficients before the first
 // barrier (latency hidden by the barrier) or at their use.
-template <bool FCT, int WPE, bool PF>
+template <bool FCT, int WPE, bool PF, bool Q16>
 __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
     k_corr(const DomDesc *__restrict__ DD, int nd, int pass, unsigned nwork)
 {
@@ -424,7 +424,8 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
     const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
     const int gi = tx * CTI - CH + lx;
     const int j0 = ty * CTJ - CH;  // global row of staged row 0
-    const SpecPass p = D.pass[pass - 1][s];
+    SpecPass p = D.pass[pass - 1][s];
+    if (!Q16) p.kind0 = p.kindo = ICAR_ADV_STORE_F32;  // an FP32-only batch: no 16-bit code
     const bool same = p.q1 == nullptr;
     const unsigned tstep = is_sr(p.kindo) ? *D.step : 0u;
     const float eps = 1e-10f, eps15 = 1e-15f;
@@ -869,6 +870,7 @@ struct FastBatch {
     bool tuned = false;  // launch configuration chosen (autotuned or fixed by the environment)
     bool dirty = true;   // device tables out of date
     bool has_sr = false; // a species is stored with stochastic rounding
+    bool q16 = false;    // a species is stored in 16 bits
 };
 
 namespace {
@@ -1050,19 +1052,28 @@ int upload_tables(FastBatch *b, std::string *err)
     return 0;
 }
 
+template <bool FCT, bool Q16>
+void launch_corr_t(unsigned nb, hipStream_t s, const DomDesc *D, int n, int pass, unsigned nw,
+                   int cfg)
+{
+    switch (cfg) {
+    case 0: hipLaunchKernelGGL((k_corr<FCT, 2, true, Q16>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+    case 1: hipLaunchKernelGGL((k_corr<FCT, 4, true, Q16>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+    case 2: hipLaunchKernelGGL((k_corr<FCT, 2, false, Q16>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+    default: hipLaunchKernelGGL((k_corr<FCT, 4, false, Q16>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+    }
+}
+
 void launch_corr(const FastBatch *b, hipStream_t s, const DomDesc *D, int n, int pass, int cfg)
 {
     const unsigned nw = b->total_work;
     const unsigned nb = (nw + 7) / 8 * 8;
-    switch ((b->fct ? 4 : 0) + cfg) {
-    case 0: hipLaunchKernelGGL((k_corr<false, 2, true>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-    case 1: hipLaunchKernelGGL((k_corr<false, 4, true>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-    case 2: hipLaunchKernelGGL((k_corr<false, 2, false>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-    case 3: hipLaunchKernelGGL((k_corr<false, 4, false>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-    case 4: hipLaunchKernelGGL((k_corr<true, 2, true>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-    case 5: hipLaunchKernelGGL((k_corr<true, 4, true>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-    case 6: hipLaunchKernelGGL((k_corr<true, 2, false>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-    default: hipLaunchKernelGGL((k_corr<true, 4, false>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+    if (b->fct) {
+        if (b->q16) launch_corr_t<true, true>(nb, s, D, n, pass, nw, cfg);
+        else launch_corr_t<true, false>(nb, s, D, n, pass, nw, cfg);
+    } else {
+        if (b->q16) launch_corr_t<false, true>(nb, s, D, n, pass, nw, cfg);
+        else launch_corr_t<false, false>(nb, s, D, n, pass, nw, cfg);
     }
 }
 
@@ -1230,9 +1241,11 @@ FastBatch *fast_batch_create(const FastDomain *doms, int n, int order, int fct, 
         D.sf = doms[k].sf;
         D.coef = st->coef;
         D.step = st->d_step;
-        for (const FastSpecies &f : st->sp)
+        for (const FastSpecies &f : st->sp) {
             b->has_sr = b->has_sr || f.storage == ICAR_ADV_STORE_BF16_SR ||
                         f.storage == ICAR_ADV_STORE_F16_SR;
+            b->q16 = b->q16 || f.storage != ICAR_ADV_STORE_F32;
+        }
         cb += cell_blocks(d);
         b->cells += (double)st->N;
         b->cell_species += (double)st->N * S;
