@@ -23,6 +23,7 @@ STORE_NAMES = {0: "f32", 1: "bf16", 2: "f16", 3: "bf16-sr", 4: "f16-sr"}
 MAX_SPECIES = 48
 MASK_ALL_FIELDS = 0x1FF
 MASK_ALL_SPECIES = 0xFFFFFFFFFFFF0000
+ECOURANT = -6
 
 
 def mask_species(s: int) -> int:
@@ -37,7 +38,8 @@ EXPORTS = (
     "icar_adv_kernel_stats", "icar_adv_set_graph", "icar_adv_ensemble_create",
     "icar_adv_ensemble_destroy", "icar_adv_ensemble_run", "icar_adv_ensemble_sync",
     "icar_adv_ensemble_last_error", "icar_adv_ensemble_profile", "icar_adv_ensemble_kernel_stats",
-    "icar_adv_launch_config", "icar_adv_ensemble_launch_config",
+    "icar_adv_launch_config", "icar_adv_ensemble_launch_config", "icar_adv_balance_w",
+    "icar_adv_compute_dt",
 )
 
 
@@ -102,6 +104,8 @@ def load() -> ctypes.CDLL:
         "icar_adv_ensemble_profile": ([vp, ip], ip),
         "icar_adv_ensemble_kernel_stats": ([vp, ip, dp, ctypes.POINTER(ctypes.c_longlong), dp], ip),
         "icar_adv_launch_config": ([vp, i3, i3], ip),
+        "icar_adv_balance_w": ([vp], ip),
+        "icar_adv_compute_dt": ([vp, ctypes.c_float, ip, fp, fp], ip),
         "icar_adv_ensemble_launch_config": ([vp, i3, i3], ip),
     }
     for name, (args, res) in sig.items():

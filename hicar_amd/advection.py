@@ -106,6 +106,24 @@ class MpdataAdvection:
             self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
+    def balance_w(self) -> None:
+        """balance_uvw (wind.f90:88-164) on the device copies of the winds:
+        the device w becomes the continuity-balanced one; ``download(
+        fields=True)`` returns it to the host."""
+        self._check(self.lib.icar_adv_balance_w(self.h))
+
+    def compute_dt(self, cfl: float, cfl_strictness: int, dz_levels) -> float:
+        """compute_dt (time_step.f90:51-168, use_density=.false.) on the device
+        winds; raises IcarAdvError (code ECOURANT) when dt < 0.1 s, where the
+        reference stops."""
+        dzl = np.ascontiguousarray(dz_levels, dtype=np.float32)
+        if dzl.shape != (self.domain.nz,):
+            raise ValueError(f"dz_levels needs {self.domain.nz} values")
+        dt = ctypes.c_float()
+        self._check(self.lib.icar_adv_compute_dt(self.h, ctypes.c_float(cfl), int(cfl_strictness),
+                                                 _ptr(dzl), ctypes.byref(dt)))
+        return dt.value
+
     def set_graph(self, enable: bool) -> None:
         """Record the run() loop in a HIP graph and replay it."""
         self._check(self.lib.icar_adv_set_graph(self.h, int(enable)))
@@ -206,6 +224,27 @@ def adv_init(domain: Domain, options: Options) -> MpdataAdvection | None:
     if options.physics.advection == kADV_STD:
         raise NotImplementedError("adv=1 (standard advection) is not part of this library")
     raise ValueError(f"unknown advection scheme {options.physics.advection}")
+
+
+def balance_uvw(domain: Domain, options: Options) -> None:
+    """wind.f90:88-164 (update=.false.) for a domain whose MPDATA state
+    adv_init created: w is rebalanced on the device from the device u, v,
+    density and metric (time_step.f90:324 runs it before advect when
+    advect_density is set)."""
+    st = getattr(domain, "_mpdata", None)
+    if st is None:
+        raise RuntimeError("adv_init(domain, options) has not been called")
+    st.balance_w()
+
+
+def compute_dt(domain: Domain, options: Options, dz_levels) -> float:
+    """update_dt's compute_dt (time_step.f90:213-246) on the device winds,
+    with options%time_options%cfl_reduction_factor and cfl_strictness."""
+    st = getattr(domain, "_mpdata", None)
+    if st is None:
+        raise RuntimeError("adv_init(domain, options) has not been called")
+    return st.compute_dt(options.time_options.cfl_reduction_factor,
+                         options.time_options.cfl_strictness, dz_levels)
 
 
 def advect(domain: Domain, options: Options, dt: float, sync_host: bool = True) -> None:

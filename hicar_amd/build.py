@@ -31,6 +31,7 @@ COMMON = [
 UNITS = [
     ("kernels_exact.hip", ["-ffp-contract=off"]),
     ("kernels_fast.hip", ["-ffp-contract=off"]),
+    ("kernels_dyn.hip", ["-ffp-contract=off"]),
     ("api.hip", ["-ffp-contract=off"]),
 ]
 

@@ -8,6 +8,8 @@
 // fused variant advances all of them per launch with identical results.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -57,6 +59,7 @@ struct Handle {
     bool have_dt = false;
     double *d_partial = nullptr;
     std::vector<double> h_partial;
+    float *d_dt = nullptr;  // dz_levels (nz) then the dt partial maxima (4 per block)
     std::string err;
     FastState *fast = nullptr;
     FastBatch *batch = nullptr;  // this domain as a batch of one (fused path)
@@ -195,6 +198,7 @@ void free_all(Handle *h)
     for (float *p : bufs) (void)hipFree(p);
     for (auto &s : h->sp) (void)hipFree(s.dev);
     (void)hipFree(h->d_partial);
+    (void)hipFree(h->d_dt);
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     fast_batch_destroy(h->batch);
     fast_destroy(h->fast);
@@ -657,6 +661,89 @@ int icar_adv_max_courant(void *handle, double out[3])
     for (int b = 0; b < nb; b++)
         for (int c = 0; c < 3; c++)
             out[c] = out[c] > h->h_partial[3 * b + c] ? out[c] : h->h_partial[3 * b + c];
+    return 0;
+}
+
+int icar_adv_balance_w(void *handle)
+{
+    Handle *h;
+    int rc = check_handle(handle, &h);
+    if (rc) return rc;
+    rc = require_bound(h);
+    if (rc) return rc;
+    // columns its-1..ite+1, jts-1..jte+1 clipped to the domain (wind.f90:113-122);
+    // at the memory bounds, which are the domain's at its edges
+    const icar_adv_grid &g = h->g;
+    const int i_s = std::max(g.its - 1, g.ims) - g.ims, i_e = std::min(g.ite + 1, g.ime) - g.ims;
+    const int j_s = std::max(g.jts - 1, g.jms) - g.jms, j_e = std::min(g.jte + 1, g.jme) - g.jms;
+    DomainFields f = domain_fields(h);
+    launch_balance_w(h->stream, h->d, i_s, i_e, j_s, j_e, g.dx, h->o.advect_density, f,
+                     h->dev_field[ICAR_ADV_FIELD_W]);
+    HIPCHK(h, hipGetLastError());
+    if (h->o.debug) HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int icar_adv_compute_dt(void *handle, float cfl, int cfl_strictness, const float *dz_levels,
+                        float *dt)
+{
+    Handle *h;
+    int rc = check_handle(handle, &h);
+    if (rc) return rc;
+    if (!dz_levels || !dt) return set_err(h, ICAR_ADV_EINVAL, "NULL dz_levels or dt");
+    rc = require_bound(h);
+    if (rc) return rc;
+    const icar_adv_grid &g = h->g;
+    const Dims &d = h->d;
+    const int its = std::max(g.its, g.ims) - g.ims, ite = std::min(g.ite, g.ime) - g.ims;
+    const int jts = std::max(g.jts, g.jms) - g.jms, jte = std::min(g.jte, g.jme) - g.jms;
+    if (its > ite || jts > jte)
+        return set_err(h, ICAR_ADV_ESHAPE, "empty tile its..ite, jts..jte");
+    const int nb = dt_partial_blocks();
+    if (!h->d_dt) {
+        rc = dalloc(h, (void **)&h->d_dt, sizeof(float) * ((size_t)d.nz + 4 * (size_t)nb));
+        if (rc) return rc;
+    }
+    float *part = h->d_dt + d.nz;
+    HIPCHK(h, hipMemcpyAsync(h->d_dt, dz_levels, sizeof(float) * d.nz, hipMemcpyHostToDevice,
+                             h->stream));
+    timed(h, K_REDUCE, 4.0 * 2 * cells(h), [&] {
+        launch_dt_partials(h->stream, d, its, ite, jts, jte, g.dx,
+                           h->dev_field[ICAR_ADV_FIELD_U], h->dev_field[ICAR_ADV_FIELD_V],
+                           h->dev_field[ICAR_ADV_FIELD_W], h->d_dt, part);
+    });
+    HIPCHK(h, hipGetLastError());
+    std::vector<float> hp(4 * (size_t)nb);
+    HIPCHK(h, hipMemcpyAsync(hp.data(), part, sizeof(float) * hp.size(), hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    float mu = 0.0f, mv = 0.0f, mw = 0.0f, m3 = 0.0f;
+    for (int b = 0; b < nb; b++) {
+        mu = std::max(mu, hp[4 * b]);
+        mv = std::max(mv, hp[4 * b + 1]);
+        mw = std::max(mw, hp[4 * b + 2]);
+        m3 = std::max(m3, hp[4 * b + 3]);
+    }
+    // time_step.f90:70-149, in FP32 as the reference computes it
+    const float three_d_cfl = 0.577350269f;
+    const float sqrt3 = sqrtf(3.0f) * 1.001f;
+    float maxwind3d;
+    if (cfl_strictness == 1) {
+        maxwind3d = std::max(std::max(mu, mv), mw) * sqrt3;
+    } else if (cfl_strictness == 5) {
+        maxwind3d = mu + mv + mw;
+    } else {
+        maxwind3d = m3;
+        if (cfl_strictness == 2)
+            maxwind3d = std::max(std::max(std::max(mu, mv), mw), maxwind3d * three_d_cfl);
+        else if (cfl_strictness == 4)
+            maxwind3d = maxwind3d * sqrt3;
+    }
+    *dt = cfl / maxwind3d;
+    if (*dt < 1e-1f)  // :158-166, where the reference stops
+        return set_err(h, ICAR_ADV_ECOURANT,
+                       "time step too small: dt = %g (Umax = %g, Vmax = %g, Wmax = %g)",
+                       (double)*dt, (double)mu, (double)mv, (double)mw);
     return 0;
 }
 

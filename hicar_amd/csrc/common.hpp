@@ -67,6 +67,14 @@ void launch_fct(hipStream_t s, Dims d, const float *q0, const float *q1, const f
 void launch_bc(hipStream_t s, Dims d, float *q, int bc_x, int bc_y);
 // FP64 mass budget partial sums: writes nblocks doubles to partial.
 int mass_partials_blocks(Dims d);
+
+// ---- wind balance and dt bound (kernels_dyn.hip) ----
+void launch_balance_w(hipStream_t s, Dims d, int i_s, int i_e, int j_s, int j_e, float dx,
+                      int adv_den, const DomainFields &f, float *w);
+int dt_partial_blocks();
+void launch_dt_partials(hipStream_t s, Dims d, int its, int ite, int jts, int jte, float dx,
+                        const float *u, const float *v, const float *w, const float *dzl,
+                        float *part);
 void launch_mass(hipStream_t s, Dims d, const float *q, const float *jaco, const float *density,
                  const float *dz, double *partial);
 // FP64 Courant maxima partials: 3 doubles per block.

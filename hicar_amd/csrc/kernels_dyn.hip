@@ -1,0 +1,176 @@
+// The two wind-field steps ICAR runs next to advect, on the device copies of
+// the winds the advection handle already holds:
+//   k_balance_w   balance_uvw (src/physics/wind.f90:88-164, update=.false.):
+//                 the horizontal mass-flux divergence (calc_divergence with
+//                 horz_only, :201-234) and w integrated up each column so
+//                 that it balances (calc_w, :166-199).  One thread per
+//                 column, threads along i (coalesced rows), the level below
+//                 carried in registers.
+//   k_dt_partials compute_dt (src/main/time_step.f90:51-168) with
+//                 use_density=.false.: per block the maxima of |u|, |v|, |w|
+//                 over the whole arrays and of the tile's per-cell Courant
+//                 rate; the host takes the maximum over blocks and finishes
+//                 in FP32 as the reference does (a maximum is exact in any
+//                 order, so the result is the reference's bits).
+// Every expression keeps the Fortran's evaluation order; this unit is
+// compiled with -ffp-contract=off.
+#include "common.hpp"
+
+namespace icaradv {
+
+namespace {
+
+// u_met of x face fi (0..nx) of row (k, j), wind.f90:222-228
+__device__ __forceinline__ float u_met(const Dims &d, int fi, int k, int j, int adv_den,
+                                       const float *__restrict__ u, const float *__restrict__ ju,
+                                       const float *__restrict__ rho)
+{
+    const size_t f = (size_t)fi + (size_t)(d.nx + 1) * ((size_t)k + (size_t)d.nz * j);
+    if (!adv_den) return u[f] * ju[f];
+    const size_t c = ix(d, 0, k, j);
+    if (fi == 0) return u[f] * ju[f] * (1.5f * rho[c] - 0.5f * rho[c + 1]);
+    if (fi == d.nx) return u[f] * ju[f] * (1.5f * rho[c + d.nx - 1] - 0.5f * rho[c + d.nx - 2]);
+    return u[f] * ju[f] * (rho[c + fi - 1] + rho[c + fi]) / 2.0f;
+}
+
+// v_met of y face fj (0..ny) of column (i, k), wind.f90:225-227
+__device__ __forceinline__ float v_met(const Dims &d, int i, int k, int fj, int adv_den,
+                                       const float *__restrict__ v, const float *__restrict__ jv,
+                                       const float *__restrict__ rho)
+{
+    const size_t f = ix(d, i, k, fj);
+    if (!adv_den) return v[f] * jv[f];
+    const size_t sj = (size_t)d.nx * d.nz;
+    const size_t c0 = ix(d, i, k, 0);
+    if (fj == 0) return v[f] * jv[f] * (1.5f * rho[c0] - 0.5f * rho[c0 + sj]);
+    if (fj == d.ny)
+        return v[f] * jv[f] * (1.5f * rho[c0 + (d.ny - 1) * sj] - 0.5f * rho[c0 + (d.ny - 2) * sj]);
+    return v[f] * jv[f] * (rho[f - sj] + rho[f]) / 2.0f;
+}
+
+__global__ void __launch_bounds__(256)
+    k_balance_w(Dims d, int i_s, int i_e, int j_s, int j_e, float dx, int adv_den, DomainFields f,
+                float *__restrict__ w)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    if (i >= d.nx || j >= d.ny) return;
+    const size_t sk = (size_t)d.nx;
+    const size_t c0 = ix(d, i, 0, j);
+    if (i < i_s || i > i_e || j < j_s || j > j_e) {  // w = 0 outside the balanced columns (:179)
+        for (int k = 0; k < d.nz; k++) w[c0 + sk * k] = 0.0f;
+        return;
+    }
+    const float *__restrict__ rho = f.density;
+    const float *__restrict__ dz = f.dz;
+    const float *__restrict__ jw = f.jaco_w;
+    float wb = 0.0f, rib = 0.0f, jwb = 0.0f;  // w, rho_i and jacobian_w of the level below
+    for (int k = 0; k < d.nz; k++) {
+        const size_t c = c0 + sk * k;
+        const float dU = u_met(d, i + 1, k, j, adv_den, f.u, f.jaco_u, rho) -
+                         u_met(d, i, k, j, adv_den, f.u, f.jaco_u, rho);
+        const float dV = v_met(d, i, k, j + 1, adv_den, f.v, f.jaco_v, rho) -
+                         v_met(d, i, k, j, adv_den, f.v, f.jaco_v, rho);
+        const float div = (dU + dV) / dx;  // :232
+        float wc;
+        if (adv_den) {
+            // rho_i(k) = (rho(k)dz(k) + rho(k+1)dz(k+1)) / (dz(k)+dz(k+1)), :177
+            const float ri = (k < d.nz - 1)
+                                 ? (rho[c] * dz[c] + rho[c + sk] * dz[c + sk]) / (dz[c] + dz[c + sk])
+                                 : 0.0f;
+            if (k == 0)
+                wc = 0.0f - div * dz[c] / (jw[c] * ri);                           // :183-184
+            else if (k == d.nz - 1)
+                wc = ((wb * rib * jwb) - div * dz[c]) / (jw[c] * rho[c]);          // :185-188
+            else
+                wc = ((wb * rib * jwb) - div * dz[c]) / (jw[c] * ri);              // :189-192
+            rib = ri;
+        } else {
+            if (k == 0)
+                wc = (0.0f - div * dz[c]) / (jw[c]);                               // :195
+            else
+                wc = (wb * jwb - div * dz[c]) / (jw[c]);                           // :197-198
+        }
+        w[c] = wc;
+        wb = wc;
+        jwb = jw[c];
+    }
+}
+
+constexpr int DT_BLOCKS = 1024, DT_THREADS = 256;
+
+__device__ __forceinline__ float block_max(float v, float *sm)
+{
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = DT_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sm[threadIdx.x] = fmaxf(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const float r = sm[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(DT_THREADS)
+    k_dt_partials(Dims d, int its, int ite, int jts, int jte, float dx, const float *__restrict__ u,
+                  const float *__restrict__ v, const float *__restrict__ w,
+                  const float *__restrict__ dzl, float *__restrict__ part)
+{
+    __shared__ float sm[DT_THREADS];
+    const size_t stride = (size_t)gridDim.x * DT_THREADS;
+    const size_t t0 = (size_t)blockIdx.x * DT_THREADS + threadIdx.x;
+    const size_t nu = (size_t)(d.nx + 1) * d.nz * d.ny, nv = (size_t)d.nx * d.nz * (d.ny + 1);
+    const size_t nw = (size_t)d.nx * d.nz * d.ny;
+    float mu = 0.0f, mv = 0.0f, mw = 0.0f, m3 = 0.0f;
+    for (size_t p = t0; p < nu; p += stride) mu = fmaxf(mu, fabsf(u[p]));
+    for (size_t p = t0; p < nv; p += stride) mv = fmaxf(mv, fabsf(v[p]));
+    for (size_t p = t0; p < nw; p += stride) mw = fmaxf(mw, fabsf(w[p]));
+    // the tile, :101-126: i fastest so that neighbouring threads read a row
+    const int ti = ite - its + 1, tj = jte - jts + 1;
+    const size_t nt = (size_t)ti * d.nz * tj;
+    for (size_t p = t0; p < nt; p += stride) {
+        const int i = its + (int)(p % (size_t)ti);
+        const size_t r = p / (size_t)ti;
+        const int k = (int)(r % (size_t)d.nz);
+        const int j = jts + (int)(r / (size_t)d.nz);
+        const size_t fu = (size_t)i + (size_t)(d.nx + 1) * ((size_t)k + (size_t)d.nz * j);
+        const size_t c = ix(d, i, k, j);
+        const size_t cb = k == 0 ? c : c - (size_t)d.nx;  // zoffset, :104-108
+        const float a = fmaxf(fabsf(u[fu]), fabsf(u[fu + 1])) / dx;
+        const float b = fmaxf(fabsf(v[c]), fabsf(v[c + (size_t)d.nx * d.nz])) / dx;
+        const float e = fmaxf(fabsf(w[c]), fabsf(w[cb])) / dzl[k];
+        m3 = fmaxf(m3, fmaxf(fmaxf(a, b), e));
+    }
+    mu = block_max(mu, sm);
+    mv = block_max(mv, sm);
+    mw = block_max(mw, sm);
+    m3 = block_max(m3, sm);
+    if (threadIdx.x == 0) {
+        part[4 * blockIdx.x + 0] = mu;
+        part[4 * blockIdx.x + 1] = mv;
+        part[4 * blockIdx.x + 2] = mw;
+        part[4 * blockIdx.x + 3] = m3;
+    }
+}
+
+}  // namespace
+
+void launch_balance_w(hipStream_t s, Dims d, int i_s, int i_e, int j_s, int j_e, float dx,
+                      int adv_den, const DomainFields &f, float *w)
+{
+    hipLaunchKernelGGL(k_balance_w, dim3((d.nx + 63) / 64, (d.ny + 3) / 4), dim3(64, 4), 0, s, d,
+                       i_s, i_e, j_s, j_e, dx, adv_den, f, w);
+}
+
+int dt_partial_blocks() { return DT_BLOCKS; }
+
+void launch_dt_partials(hipStream_t s, Dims d, int its, int ite, int jts, int jte, float dx,
+                        const float *u, const float *v, const float *w, const float *dzl,
+                        float *part)
+{
+    hipLaunchKernelGGL(k_dt_partials, dim3(DT_BLOCKS), dim3(DT_THREADS), 0, s, d, its, ite, jts,
+                       jte, dx, u, v, w, dzl, part);
+}
+
+}  // namespace icaradv

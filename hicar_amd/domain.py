@@ -44,8 +44,10 @@ class Parameters:
 
 @dataclass
 class TimeOptions:
-    """``&time_parameters`` (options_obj.f90:2198)."""
+    """``&time_parameters`` (options_obj.f90:2188-2208) with its defaults."""
     RK3: bool = False
+    cfl_reduction_factor: float = 0.9
+    cfl_strictness: int = 3
 
 
 @dataclass

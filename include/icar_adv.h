@@ -154,6 +154,26 @@ int icar_adv_mass_budget(void *handle, double *per_species);
  * time_step.f90:98-127, expressed as Courant numbers).  Synchronises. */
 int icar_adv_max_courant(void *handle, double out[3]);
 
+/* = balance_uvw(domain, options) (src/physics/wind.f90:88-164, update
+ * =.false.), which the time step runs before advect when advect_density is
+ * set (src/main/time_step.f90:324): recomputes the device copy of w from
+ * the device u, v, density, jacobians and advection_dz so that the mass
+ * flux is non-divergent (calc_divergence with horz_only, :201-234, then
+ * calc_w, :166-199), over the columns its-1..ite+1 x jts-1..jte+1 clipped to
+ * the memory bounds, w = 0 elsewhere.  Uses the handle's advect_density.
+ * icar_adv_download with ICAR_ADV_MASK_FIELD(ICAR_ADV_FIELD_W) returns it. */
+int icar_adv_balance_w(void *handle);
+
+/* = compute_dt(dx, u, v, w, rho, dz_levels, ..., CFL, cfl_strictness,
+ * use_density=.false.) (src/main/time_step.f90:51-168, called by update_dt
+ * :213-246) on the device copies of u, v and w: the reference's FP32 result
+ * in *dt.  dz_levels holds the nz nominal level thicknesses
+ * (options%parameters%dz_levels); cfl is cfl_reduction_factor.  Returns
+ * ICAR_ADV_ECOURANT, with *dt set, when dt < 0.1 s, where the reference
+ * stops.  Synchronises. */
+int icar_adv_compute_dt(void *handle, float cfl, int cfl_strictness, const float *dz_levels,
+                        float *dt);
+
 /* Message of the last failure on this handle (handle may be NULL). */
 const char *icar_adv_last_error(void *handle);
 

@@ -129,6 +129,54 @@ class Oracle:
                 self.apply_bc(q, bc_x, bc_y)
         return cur
 
+    # -- wind balance and time step (dyn_oracle.c) ------------------------
+    def _winds(self, dom):
+        return [self.arr(getattr(dom, n)) for n in
+                ("u", "v", "density", "jacobian_u", "jacobian_v", "jacobian_w", "advection_dz")]
+
+    def calc_divergence(self, dom, advect_density, w=None, horz_only=True):
+        """calc_divergence (wind.f90:201-246) of the domain's winds."""
+        u, v, rho, ju, jv, jw, dz = self._winds(dom)
+        w = self.arr(dom.w if w is None else w)
+        div = np.zeros(dom.shape, self.dtype)
+        self.lib.oracle_calc_divergence(dom.nx, dom.ny, dom.nz, self.real(dom.dx),
+                                        int(advect_density), int(horz_only), self._p(u),
+                                        self._p(v), self._p(w), self._p(rho), self._p(ju),
+                                        self._p(jv), self._p(jw), self._p(dz), self._p(div))
+        return div
+
+    def balance_w(self, dom, advect_density, i_s=None, i_e=None, j_s=None, j_e=None):
+        """balance_uvw (wind.f90:88-164, update=.false.): w from the
+        horizontal divergence over columns i_s..i_e x j_s..j_e (0-based,
+        inclusive; default: every column), zero elsewhere."""
+        i_s = 0 if i_s is None else i_s
+        j_s = 0 if j_s is None else j_s
+        i_e = dom.nx - 1 if i_e is None else i_e
+        j_e = dom.ny - 1 if j_e is None else j_e
+        u, v, rho, ju, jv, jw, dz = self._winds(dom)
+        div = self.calc_divergence(dom, advect_density)
+        w = np.zeros(dom.shape, self.dtype)
+        self.lib.oracle_calc_w(dom.nx, dom.ny, dom.nz, i_s, i_e, j_s, j_e, int(advect_density),
+                               self._p(div), self._p(dz), self._p(jw), self._p(rho), self._p(w))
+        return w
+
+    def compute_dt(self, dom, dz_levels, cfl, strictness, its=None, ite=None, jts=None,
+                   jte=None, w=None):
+        """compute_dt (time_step.f90:51-168, use_density=.false.) over the
+        tile its..ite x jts..jte (0-based, inclusive; default: all cells)."""
+        its = 0 if its is None else its
+        jts = 0 if jts is None else jts
+        ite = dom.nx - 1 if ite is None else ite
+        jte = dom.ny - 1 if jte is None else jte
+        u, v = self.arr(dom.u), self.arr(dom.v)
+        w = self.arr(dom.w if w is None else w)
+        dzl = self.arr(dz_levels)
+        assert dzl.shape == (dom.nz,)
+        self.lib.oracle_compute_dt.restype = self.real
+        return self.lib.oracle_compute_dt(dom.nx, dom.ny, dom.nz, its, ite, jts, jte,
+                                          self.real(dom.dx), self._p(u), self._p(v), self._p(w),
+                                          self._p(dzl), self.real(cfl), int(strictness))
+
     def divergence(self, U, V, W, dz):
         nx, ny, nz = self._dims(U)
         mx = ctypes.c_double()
