@@ -17,12 +17,15 @@ module icar_adv_c
     public :: icar_adv_upload, icar_adv_download, icar_adv_advect, icar_adv_mass_budget
     public :: icar_adv_max_courant, icar_adv_last_error, icar_adv_sync, icar_adv_apply_bc
     public :: icar_adv_run, icar_adv_set_graph, icar_adv_error_message
+    public :: icar_adv_balance_w, icar_adv_compute_dt, icar_adv_launch_config
     public :: ICAR_ADV_FIELD_U, ICAR_ADV_FIELD_V, ICAR_ADV_FIELD_W, ICAR_ADV_FIELD_DENSITY
     public :: ICAR_ADV_FIELD_JACOBIAN, ICAR_ADV_FIELD_JACOBIAN_U, ICAR_ADV_FIELD_JACOBIAN_V
     public :: ICAR_ADV_FIELD_JACOBIAN_W, ICAR_ADV_FIELD_ADVECTION_DZ
     public :: ICAR_ADV_STORE_F32, ICAR_ADV_STORE_BF16, ICAR_ADV_STORE_F16
     public :: ICAR_ADV_STORE_BF16_SR, ICAR_ADV_STORE_F16_SR
     public :: ICAR_ADV_MASK_ALL_FIELDS, ICAR_ADV_MASK_ALL_SPECIES
+    public :: ICAR_ADV_OK, ICAR_ADV_EINVAL, ICAR_ADV_ESHAPE, ICAR_ADV_EUNBOUND, ICAR_ADV_ERANGE
+    public :: ICAR_ADV_ENOMEM, ICAR_ADV_ECOURANT
 
     integer(c_int), parameter :: ICAR_ADV_FIELD_U = 0, ICAR_ADV_FIELD_V = 1, ICAR_ADV_FIELD_W = 2
     integer(c_int), parameter :: ICAR_ADV_FIELD_DENSITY = 3, ICAR_ADV_FIELD_JACOBIAN = 4
@@ -30,6 +33,9 @@ module icar_adv_c
     integer(c_int), parameter :: ICAR_ADV_FIELD_JACOBIAN_W = 7, ICAR_ADV_FIELD_ADVECTION_DZ = 8
     integer(c_int), parameter :: ICAR_ADV_STORE_F32 = 0, ICAR_ADV_STORE_BF16 = 1, ICAR_ADV_STORE_F16 = 2
     integer(c_int), parameter :: ICAR_ADV_STORE_BF16_SR = 3, ICAR_ADV_STORE_F16_SR = 4
+    integer(c_int), parameter :: ICAR_ADV_OK = 0, ICAR_ADV_EINVAL = -1, ICAR_ADV_ESHAPE = -2
+    integer(c_int), parameter :: ICAR_ADV_EUNBOUND = -3, ICAR_ADV_ERANGE = -4, ICAR_ADV_ENOMEM = -5
+    integer(c_int), parameter :: ICAR_ADV_ECOURANT = -6
     integer(c_long_long), parameter :: ICAR_ADV_MASK_ALL_FIELDS = int(z'1FF', c_long_long)
     ! bits 16..63 (species 0..47); written as a negative two's-complement value
     integer(c_long_long), parameter :: ICAR_ADV_MASK_ALL_SPECIES = -65536_c_long_long
@@ -140,6 +146,30 @@ module icar_adv_c
             import :: c_int, c_ptr
             type(c_ptr), value :: handle
             integer(c_int), value :: enable
+        end function
+
+        !> balance_uvw (wind.f90:88-164) on the device winds
+        integer(c_int) function icar_adv_balance_w(handle) bind(C, name="icar_adv_balance_w")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+        end function
+
+        !> compute_dt (time_step.f90:51-168, use_density=.false.) on the device winds
+        integer(c_int) function icar_adv_compute_dt(handle, cfl, cfl_strictness, dz_levels, dt) &
+                bind(C, name="icar_adv_compute_dt")
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: handle
+            real(c_float), value :: cfl
+            integer(c_int), value :: cfl_strictness
+            real(c_float), intent(in) :: dz_levels(*)
+            real(c_float), intent(out) :: dt
+        end function
+
+        integer(c_int) function icar_adv_launch_config(handle, corr_cfg, ksplit) &
+                bind(C, name="icar_adv_launch_config")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+            integer(c_int), intent(out) :: corr_cfg, ksplit
         end function
     end interface
 

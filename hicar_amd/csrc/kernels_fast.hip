@@ -46,6 +46,21 @@ namespace icaradv {
 namespace {
 
 // ---------------------------------------------------------------------------
+// Pointers read from the descriptor tables in device memory are generic to
+// the compiler, which would then emit flat loads (64-bit per-lane addresses,
+// both memory counters).  Every buffer here is device memory, so the kernels
+// view them in the global address space: global loads with a scalar base
+// and a 32-bit lane offset.
+// ---------------------------------------------------------------------------
+typedef __attribute__((address_space(1))) const float gcfloat;
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) const uint16_t gcu16;
+typedef __attribute__((address_space(1))) uint16_t gu16;
+
+__device__ __forceinline__ gcfloat *gp(const float *p) { return (gcfloat *)p; }
+__device__ __forceinline__ gfloat *gp(float *p) { return (gfloat *)p; }
+
+// ---------------------------------------------------------------------------
 // species storage: FP32 or 16-bit (BF16 / FP16) holding value * 2^-e
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool is_bf16(int kind)
@@ -55,8 +70,8 @@ __device__ __forceinline__ bool is_bf16(int kind)
 
 __device__ __forceinline__ float ld_q(const void *p, size_t i, int kind, float scale)
 {
-    if (kind == ICAR_ADV_STORE_F32) return static_cast<const float *>(p)[i];
-    const uint16_t b = static_cast<const uint16_t *>(p)[i];
+    if (kind == ICAR_ADV_STORE_F32) return ((gcfloat *)p)[i];
+    const uint16_t b = ((gcu16 *)p)[i];
     const float v = is_bf16(kind) ? __uint_as_float((uint32_t)b << 16)
                                   : __half2float(__ushort_as_half(b));
     return v * scale;
@@ -112,9 +127,9 @@ __device__ __forceinline__ void st_q(void *p, size_t i, int kind, float inv_scal
                                      uint32_t rnd)
 {
     if (kind == ICAR_ADV_STORE_F32)
-        static_cast<float *>(p)[i] = v;
+        ((gfloat *)p)[i] = v;
     else
-        static_cast<uint16_t *>(p)[i] = to16(v, kind, inv_scale, rnd);
+        ((gu16 *)p)[i] = to16(v, kind, inv_scale, rnd);
 }
 
 __device__ __forceinline__ bool is_sr(int kind)
@@ -198,38 +213,43 @@ __global__ void __launch_bounds__(256) k_courant_b(const DomDesc *__restrict__ D
     cell_of(D, blockIdx.x, i, k, j);
     const Dims d = D.d;
     if (i >= d.nx || j >= d.ny) return;
-    const DomainFields &f = D.f;
-    const StepFields &o = D.sf;
+    gcfloat *__restrict__ den = gp(D.f.density), *__restrict__ fu = gp(D.f.u);
+    gcfloat *__restrict__ fv = gp(D.f.v), *__restrict__ fw = gp(D.f.w);
+    gcfloat *__restrict__ ju = gp(D.f.jaco_u), *__restrict__ jv = gp(D.f.jaco_v);
+    gcfloat *__restrict__ jw = gp(D.f.jaco_w), *__restrict__ jc = gp(D.f.jaco);
+    gcfloat *__restrict__ dz = gp(D.f.dz);
+    gfloat *__restrict__ oU = gp(D.sf.U), *__restrict__ oV = gp(D.sf.V), *__restrict__ oW = gp(D.sf.W);
+    gfloat *__restrict__ oG = gp(D.sf.G), *__restrict__ oGz = gp(D.sf.Gz), *__restrict__ oWd = gp(D.sf.Wd);
     const float dt = D.dt, dx = D.dx;
     const int advect_density = D.advect_density;
     const size_t c = ix(d, i, k, j);
     const size_t sj = (size_t)d.nx * d.nz;
-    const float rc = advect_density ? f.density[c] : 1.0f;  // :489-490
+    const float rc = advect_density ? den[c] : 1.0f;  // :489-490
     if (i >= 1) {
         const size_t iu = (size_t)i + (size_t)(d.nx + 1) * ((size_t)k + (size_t)d.nz * j);
-        const float rl = advect_density ? f.density[c - 1] : 1.0f;
-        o.U[c] = f.u[iu] * dt * (rc + rl) * 0.5f * f.jaco_u[iu] / dx;  // :492-493
+        const float rl = advect_density ? den[c - 1] : 1.0f;
+        oU[c] = fu[iu] * dt * (rc + rl) * 0.5f * ju[iu] / dx;  // :492-493
     } else {
-        o.U[c] = 0.0f;
+        oU[c] = 0.0f;
     }
     if (j >= 1) {
-        const float rs = advect_density ? f.density[c - sj] : 1.0f;
-        o.V[c] = f.v[c] * dt * (rc + rs) * 0.5f * f.jaco_v[c] / dx;  // :494-495
+        const float rs = advect_density ? den[c - sj] : 1.0f;
+        oV[c] = fv[c] * dt * (rc + rs) * 0.5f * jv[c] / dx;  // :494-495
     } else {
-        o.V[c] = 0.0f;
+        oV[c] = 0.0f;
     }
     float Wc;
     if (k < d.nz - 1) {
-        const float ru = advect_density ? f.density[c + d.nx] : 1.0f;
-        Wc = f.w[c] * dt * f.jaco_w[c] * (ru + rc) * 0.5f;  // :496-497
+        const float ru = advect_density ? den[c + d.nx] : 1.0f;
+        Wc = fw[c] * dt * jw[c] * (ru + rc) * 0.5f;  // :496-497
     } else {
-        Wc = f.w[c] * dt * f.jaco_w[c] * rc;  // :498
+        Wc = fw[c] * dt * jw[c] * rc;  // :498
     }
-    o.W[c] = Wc;
-    const float jac = f.jaco[c], den = f.density[c], dzc = f.dz[c];
-    o.G[c] = jac * den;
-    o.Gz[c] = dzc * jac * den;
-    o.Wd[c] = Wc / dzc;
+    oW[c] = Wc;
+    const float jac = jc[c], dnc = den[c], dzc = dz[c];
+    oG[c] = jac * dnc;
+    oGz[c] = dzc * jac * dnc;
+    oWd[c] = Wc / dzc;
 }
 
 // ---------------------------------------------------------------------------
@@ -256,9 +276,9 @@ __global__ void __launch_bounds__(256) k_coef(const DomDesc *__restrict__ DD, in
     cell_of(D, blockIdx.x, i, k, j);
     const Dims d = D.d;
     if (i >= d.nx || j >= d.ny) return;
-    const float *__restrict__ U = D.sf.U, *__restrict__ V = D.sf.V, *__restrict__ Wd = D.sf.Wd;
-    const float *__restrict__ G = D.sf.G;
-    float *__restrict__ coef = D.coef;
+    gcfloat *__restrict__ U = gp(D.sf.U), *__restrict__ V = gp(D.sf.V), *__restrict__ Wd = gp(D.sf.Wd);
+    gcfloat *__restrict__ G = gp(D.sf.G);
+    gfloat *__restrict__ coef = gp(D.coef);
     const int nx = d.nx, ny = d.ny, nz = d.nz;
     const size_t N = (size_t)nx * ny * nz;
     const size_t sk = (size_t)nx, sj = (size_t)nx * nz;
@@ -293,10 +313,10 @@ __global__ void __launch_bounds__(256) k_coef(const DomDesc *__restrict__ DD, in
         if (j >= 1 && j <= ny - 2)
             v[CZV] = 0.5f * Wc * ((1.0f / 4.0f) * (V[c] + V[c + sk] + V[c + sj] + V[c + sk + sj]));
     }
-    float4 *g = reinterpret_cast<float4 *>(coef);
-    g[c] = make_float4(v[CUX], v[CXV], v[CXW], v[SGX]);
-    g[N + c] = make_float4(v[CUY], v[CYU], v[CYW], v[SGY]);
-    g[2 * N + c] = make_float4(v[CUZ], v[CZU], v[CZV], v[SGZ]);
+#pragma unroll
+    for (int g = 0; g < 3; g++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) coef[4 * ((size_t)g * N + c) + e] = v[4 * g + e];
 }
 
 // one step taken by every domain of the batch (a vector store per domain)
@@ -325,8 +345,8 @@ __global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ 
     cell_of(D, blockIdx.x, i, k, j);
     const Dims d = D.d;
     if (i >= d.nx || j >= d.ny) return;
-    const float *__restrict__ U = D.sf.U, *__restrict__ V = D.sf.V, *__restrict__ W = D.sf.W;
-    const float *__restrict__ G = D.sf.G, *__restrict__ Gz = D.sf.Gz;
+    gcfloat *__restrict__ U = gp(D.sf.U), *__restrict__ V = gp(D.sf.V), *__restrict__ W = gp(D.sf.W);
+    gcfloat *__restrict__ G = gp(D.sf.G), *__restrict__ Gz = gp(D.sf.Gz);
     const SpecPass *__restrict__ sp = D.pass[0];
     const int nspec = D.nspec;
     const size_t sk = (size_t)d.nx, sj = (size_t)d.nx * d.nz;
@@ -429,10 +449,13 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
     const bool same = p.q1 == nullptr;
     const unsigned tstep = is_sr(p.kindo) ? *D.step : 0u;
     const float eps = 1e-10f, eps15 = 1e-15f;
-    const float *__restrict__ Gp = D.sf.G, *__restrict__ Gzp = D.sf.Gz, *__restrict__ dzp = D.f.dz;
-    const float4 *__restrict__ gx = reinterpret_cast<const float4 *>(D.coef);
-    const float4 *__restrict__ gy = gx + N;
-    const float4 *__restrict__ gz = gx + 2 * N;
+    gcfloat *__restrict__ Gp = gp(D.sf.G), *__restrict__ Gzp = gp(D.sf.Gz), *__restrict__ dzp = gp(D.f.dz);
+    gcfloat *__restrict__ q1p = gp(p.q1);
+    // three groups of four coefficients per cell (x, y, z faces), one
+    // 128-bit load each
+    gcfloat *__restrict__ gx = gp(D.coef);
+    gcfloat *__restrict__ gy = gx + 4 * N;
+    gcfloat *__restrict__ gz = gx + 8 * N;
 
     __shared__ float sq1[4][CY][CX];  // q1 at levels m-3..m (ring by level & 3)
     __shared__ float sq0[3][CY][CX];  // q0 at levels m-2..m (ring by level % 3)
@@ -484,7 +507,7 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
         if (row_in[r]) {
             const unsigned c = rowbase[r] + sk * (unsigned)m0;
             n0[r] = ld_q(p.q0, c, p.kind0, p.scale0);
-            n1[r] = same ? n0[r] : p.q1[c];
+            n1[r] = same ? n0[r] : q1p[c];
         }
     }
 
@@ -513,7 +536,7 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                 if (row_in[r]) {
                     const unsigned c = rowbase[r] + sk * (unsigned)(m + 1);
                     n0[r] = ld_q(p.q0, c, p.kind0, p.scale0);
-                    n1[r] = same ? n0[r] : p.q1[c];
+                    n1[r] = same ? n0[r] : q1p[c];
                 }
             }
         }
@@ -531,7 +554,8 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                 if (own_in[t]) {
                     if (L2 >= 0) {
                         const unsigned c = own_base[t] + sk * (unsigned)L2;
-                        const float4 v = gx[c];
+                        gcfloat *v4 = gx + 4 * (size_t)c;
+                        const float4 v = make_float4(v4[0], v4[1], v4[2], v4[3]);
                         kx[t][0] = v.x;
                         kx[t][1] = v.y;
                         kx[t][2] = v.z;
@@ -541,7 +565,8 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                     }
                     if (L1 >= 0 && L1 <= nz - 2) {
                         const unsigned c = own_base[t] + sk * (unsigned)L1;
-                        const float4 v = gz[c];
+                        gcfloat *v4 = gz + 4 * (size_t)c;
+                        const float4 v = make_float4(v4[0], v4[1], v4[2], v4[3]);
                         kz[t][0] = v.x;
                         kz[t][1] = v.y;
                         kz[t][2] = v.z;
@@ -556,7 +581,8 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                 for (int e = 0; e < 4; e++) ky[r][e] = 0.0f;
                 if (row_in[r] && L2 >= 0) {
                     const unsigned c = rowbase[r] + sk * (unsigned)L2;
-                    const float4 v = gy[c];
+                    gcfloat *v4 = gy + 4 * (size_t)c;
+                        const float4 v = make_float4(v4[0], v4[1], v4[2], v4[3]);
                     ky[r][0] = v.x;
                     ky[r][1] = v.y;
                     ky[r][2] = v.z;
