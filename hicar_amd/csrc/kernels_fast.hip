@@ -330,6 +330,7 @@ __global__ void __launch_bounds__(64) k_tick(const DomDesc *__restrict__ DD, int
 // donor-cell pass for every species (upwind_advection, adv_mpdata.f90:43-103)
 // ---------------------------------------------------------------------------
 constexpr int FBX = 64, FBY = 4;
+constexpr int FSB = 4;  // species per load group in k_first
 
 // flux1 (adv_mpdata.f90:39)
 __device__ __forceinline__ float flux1(float l, float r, float U)
@@ -337,6 +338,7 @@ __device__ __forceinline__ float flux1(float l, float r, float U)
     return ((U + fabsf(U)) * l + (U - fabsf(U)) * r) / 2.0f;
 }
 
+template <bool Q16>
 __global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ DD, int nd)
 {
     const int dm = find_dom_cb(DD, nd, blockIdx.x);
@@ -364,33 +366,51 @@ __global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ 
         gz = Gz[c];
     }
     const unsigned tstep = *D.step;
-    for (int s = 0; s < nspec; s++) {
-        const SpecPass p = sp[s];
-        const uint32_t rnd = is_sr(p.kindo) ? sr_bits((uint32_t)c, s, tstep) : 0u;
-        const float qc = ld_q(p.q0, c, p.kind0, p.scale0);
-        if (ring) {  // :61-63
-            st_q(p.out, c, p.kindo, p.inv_scaleo, qc, rnd);
-            continue;
+    // Species in groups of FSB: every load of a group is issued before the
+    // first store, so a thread keeps several species' reads in flight (the
+    // compiler cannot move loads above the stores of other species itself:
+    // the table's pointers may alias).
+    for (int s0 = 0; s0 < nspec; s0 += FSB) {
+        float qc[FSB], qw[FSB], qe[FSB], qs[FSB], qn[FSB], qu[FSB], qd[FSB];
+#pragma unroll
+        for (int b = 0; b < FSB; b++) {
+            qc[b] = qw[b] = qe[b] = qs[b] = qn[b] = qu[b] = qd[b] = 0.0f;
+            if (s0 + b < nspec) {
+                SpecPass p = sp[s0 + b];
+                if (!Q16) p.kind0 = ICAR_ADV_STORE_F32;  // an FP32-only batch: no 16-bit code
+                qc[b] = ld_q(p.q0, c, p.kind0, p.scale0);
+                if (!ring) {
+                    qw[b] = ld_q(p.q0, c - 1, p.kind0, p.scale0);
+                    qe[b] = ld_q(p.q0, c + 1, p.kind0, p.scale0);
+                    qs[b] = ld_q(p.q0, c - sj, p.kind0, p.scale0);
+                    qn[b] = ld_q(p.q0, c + sj, p.kind0, p.scale0);
+                    if (k < d.nz - 1) qu[b] = ld_q(p.q0, c + sk, p.kind0, p.scale0);
+                    if (k > 0) qd[b] = ld_q(p.q0, c - sk, p.kind0, p.scale0);
+                }
+            }
         }
-        const float qw = ld_q(p.q0, c - 1, p.kind0, p.scale0);
-        const float qe = ld_q(p.q0, c + 1, p.kind0, p.scale0);
-        const float qs = ld_q(p.q0, c - sj, p.kind0, p.scale0);
-        const float qn = ld_q(p.q0, c + sj, p.kind0, p.scale0);
-        const float fr = flux1(qc, qe, Ur), fl = flux1(qw, qc, Ul);
-        const float fn = flux1(qc, qn, Vn), fs = flux1(qs, qc, Vs);
-        float q = qc - ((fr - fl) + (fn - fs)) / g;  // :86-87
-        if (k == 0) {
-            const float qu = ld_q(p.q0, c + sk, p.kind0, p.scale0);
-            q = q - flux1(qc, qu, Wt) / gz;  // :93-94
-        } else if (k == d.nz - 1) {
-            const float qd = ld_q(p.q0, c - sk, p.kind0, p.scale0);
-            q = q - (qc * Wt - flux1(qd, qc, Wb)) / gz;  // top flux not upwinded (:96-97)
-        } else {
-            const float qu = ld_q(p.q0, c + sk, p.kind0, p.scale0);
-            const float qd = ld_q(p.q0, c - sk, p.kind0, p.scale0);
-            q = q - (flux1(qc, qu, Wt) - flux1(qd, qc, Wb)) / gz;  // :90-91
+#pragma unroll
+        for (int b = 0; b < FSB; b++) {
+            const int s = s0 + b;
+            if (s >= nspec) continue;
+            SpecPass p = sp[s];
+            if (!Q16) p.kindo = ICAR_ADV_STORE_F32;
+            const uint32_t rnd = is_sr(p.kindo) ? sr_bits((uint32_t)c, s, tstep) : 0u;
+            if (ring) {  // :61-63
+                st_q(p.out, c, p.kindo, p.inv_scaleo, qc[b], rnd);
+                continue;
+            }
+            const float fr = flux1(qc[b], qe[b], Ur), fl = flux1(qw[b], qc[b], Ul);
+            const float fn = flux1(qc[b], qn[b], Vn), fs = flux1(qs[b], qc[b], Vs);
+            float q = qc[b] - ((fr - fl) + (fn - fs)) / g;  // :86-87
+            if (k == 0)
+                q = q - flux1(qc[b], qu[b], Wt) / gz;  // :93-94
+            else if (k == d.nz - 1)
+                q = q - (qc[b] * Wt - flux1(qd[b], qc[b], Wb)) / gz;  // top flux not upwinded (:96-97)
+            else
+                q = q - (flux1(qc[b], qu[b], Wt) - flux1(qd[b], qc[b], Wb)) / gz;  // :90-91
+            st_q(p.out, c, p.kindo, p.inv_scaleo, q, rnd);
         }
-        st_q(p.out, c, p.kindo, p.inv_scaleo, q, rnd);
     }
 }
 
@@ -923,6 +943,18 @@ int build_tables(FastState *st, int order)
     const int S = (int)st->sp.size();
     const int P = order < 1 ? 1 : order;
     const size_t need = (size_t)2 * P * S;
+    // FP32 buffers of the intermediate passes: s1 from order 2 on, s2 from
+    // order 3 on (at order 1 the donor-cell pass writes the result directly)
+    if (P >= 2 && !st->s1 && hipMalloc(&st->s1, (size_t)S * st->N * sizeof(float)) != hipSuccess) {
+        st->s1 = nullptr;
+        st->err = "hipMalloc of the FP32 pass buffer failed";
+        return ICAR_ADV_ENOMEM;
+    }
+    if (P >= 3 && !st->s2 && hipMalloc(&st->s2, (size_t)S * st->N * sizeof(float)) != hipSuccess) {
+        st->s2 = nullptr;
+        st->err = "hipMalloc of the second FP32 pass buffer failed";
+        return ICAR_ADV_ENOMEM;
+    }
     if (need > st->tab_cap) {
         (void)hipFree(st->d_tab);
         st->d_tab = nullptr;
@@ -939,7 +971,8 @@ int build_tables(FastState *st, int order)
             for (int s = 0; s < S; s++) {
                 const FastSpecies &f = st->sp[s];
                 const float sc = ldexpf(1.0f, f.pow2), isc = ldexpf(1.0f, -f.pow2);
-                float *sa = st->s1 + (size_t)s * st->N, *sb = st->s2 + (size_t)s * st->N;
+                float *sa = st->s1 ? st->s1 + (size_t)s * st->N : nullptr;
+                float *sb = st->s2 ? st->s2 + (size_t)s * st->N : nullptr;
                 SpecPass e{};
                 void *hcur = home(st, s, par), *hoth = home(st, s, 1 - par);
                 if (pass == 1) {  // q0 = home
@@ -1013,12 +1046,7 @@ int fast_set_species(FastState *st, const FastSpecies *sp, int n)
             st->err = "hipMalloc of a species buffer failed";
             return ICAR_ADV_ENOMEM;
         }
-    if (hipMalloc(&st->s1, (size_t)n * st->N * sizeof(float)) != hipSuccess ||
-        hipMalloc(&st->s2, (size_t)n * st->N * sizeof(float)) != hipSuccess) {
-        st->err = "hipMalloc of the FP32 pass scratch failed";
-        return ICAR_ADV_ENOMEM;
-    }
-    return 0;
+    return 0;  // the FP32 pass buffers follow the order (build_tables)
 }
 
 void *fast_species_ptr(FastState *st, int s)
@@ -1346,7 +1374,12 @@ int fast_batch_advect(FastBatch *b, hipStream_t s, const float *dt, const TimedF
           [&] { hipLaunchKernelGGL(k_coef, dim3(b->total_cb), dim3(64, 4), 0, s, D, n); });
     // pass 1: donor cell for every species
     timed(K_FUSED_FIRST, 5.0 * 4 * cells + 2.0 * 4 * cs,
-          [&] { hipLaunchKernelGGL(k_first, dim3(b->total_cb), dim3(FBX, FBY), 0, s, D, n); });
+          [&] {
+              if (b->q16)
+                  hipLaunchKernelGGL(k_first<true>, dim3(b->total_cb), dim3(FBX, FBY), 0, s, D, n);
+              else
+                  hipLaunchKernelGGL(k_first<false>, dim3(b->total_cb), dim3(FBX, FBY), 0, s, D, n);
+          });
     for (int pass = 2; pass <= b->order; pass++) {
         // species read (q0 and q1 at pass 2) and written once; coefficients,
         // G, Gz and dz once
