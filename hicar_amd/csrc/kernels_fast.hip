@@ -68,6 +68,25 @@ __device__ __forceinline__ bool is_bf16(int kind)
     return kind == ICAR_ADV_STORE_BF16 || kind == ICAR_ADV_STORE_BF16_SR;
 }
 
+// A species value in two steps: the raw bits (a 32- or 16-bit load), then the
+// conversion.  Kernels that keep several loads in flight load raw bits first
+// and convert when the values are used, so that no conversion right after a
+// load makes the wave wait for it.
+typedef __attribute__((address_space(1))) const uint32_t gcu32;
+
+__device__ __forceinline__ uint32_t ld_raw(const void *p, size_t i, int kind)
+{
+    return kind == ICAR_ADV_STORE_F32 ? ((gcu32 *)p)[i] : (uint32_t)((gcu16 *)p)[i];
+}
+
+__device__ __forceinline__ float cvt_q(uint32_t raw, int kind, float scale)
+{
+    if (kind == ICAR_ADV_STORE_F32) return __uint_as_float(raw);
+    const float v = is_bf16(kind) ? __uint_as_float(raw << 16)
+                                  : __half2float(__ushort_as_half((uint16_t)raw));
+    return v * scale;
+}
+
 __device__ __forceinline__ float ld_q(const void *p, size_t i, int kind, float scale)
 {
     if (kind == ICAR_ADV_STORE_F32) return ((gcfloat *)p)[i];
@@ -371,23 +390,40 @@ __global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ 
     // compiler cannot move loads above the stores of other species itself:
     // the table's pointers may alias).
     for (int s0 = 0; s0 < nspec; s0 += FSB) {
-        float qc[FSB], qw[FSB], qe[FSB], qs[FSB], qn[FSB], qu[FSB], qd[FSB];
+        uint32_t rc[FSB], rw[FSB], re[FSB], rs[FSB], rn[FSB], ru[FSB], rd[FSB];
 #pragma unroll
         for (int b = 0; b < FSB; b++) {
-            qc[b] = qw[b] = qe[b] = qs[b] = qn[b] = qu[b] = qd[b] = 0.0f;
+            rc[b] = rw[b] = re[b] = rs[b] = rn[b] = ru[b] = rd[b] = 0u;
             if (s0 + b < nspec) {
                 SpecPass p = sp[s0 + b];
                 if (!Q16) p.kind0 = ICAR_ADV_STORE_F32;  // an FP32-only batch: no 16-bit code
-                qc[b] = ld_q(p.q0, c, p.kind0, p.scale0);
+                rc[b] = ld_raw(p.q0, c, p.kind0);
                 if (!ring) {
-                    qw[b] = ld_q(p.q0, c - 1, p.kind0, p.scale0);
-                    qe[b] = ld_q(p.q0, c + 1, p.kind0, p.scale0);
-                    qs[b] = ld_q(p.q0, c - sj, p.kind0, p.scale0);
-                    qn[b] = ld_q(p.q0, c + sj, p.kind0, p.scale0);
-                    if (k < d.nz - 1) qu[b] = ld_q(p.q0, c + sk, p.kind0, p.scale0);
-                    if (k > 0) qd[b] = ld_q(p.q0, c - sk, p.kind0, p.scale0);
+                    rw[b] = ld_raw(p.q0, c - 1, p.kind0);
+                    re[b] = ld_raw(p.q0, c + 1, p.kind0);
+                    rs[b] = ld_raw(p.q0, c - sj, p.kind0);
+                    rn[b] = ld_raw(p.q0, c + sj, p.kind0);
+                    if (k < d.nz - 1) ru[b] = ld_raw(p.q0, c + sk, p.kind0);
+                    if (k > 0) rd[b] = ld_raw(p.q0, c - sk, p.kind0);
                 }
             }
+        }
+        float qc[FSB], qw[FSB], qe[FSB], qs[FSB], qn[FSB], qu[FSB], qd[FSB];
+#pragma unroll
+        for (int b = 0; b < FSB; b++) {
+            int kind = ICAR_ADV_STORE_F32;
+            float sc = 1.0f;
+            if (Q16 && s0 + b < nspec) {
+                kind = sp[s0 + b].kind0;
+                sc = sp[s0 + b].scale0;
+            }
+            qc[b] = cvt_q(rc[b], kind, sc);
+            qw[b] = cvt_q(rw[b], kind, sc);
+            qe[b] = cvt_q(re[b], kind, sc);
+            qs[b] = cvt_q(rs[b], kind, sc);
+            qn[b] = cvt_q(rn[b], kind, sc);
+            qu[b] = cvt_q(ru[b], kind, sc);
+            qd[b] = cvt_q(rd[b], kind, sc);
         }
 #pragma unroll
         for (int b = 0; b < FSB; b++) {
@@ -519,15 +555,18 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
         Fzp[t] = 0.0f;
     }
 
-    // register prefetch of the next staged level
-    float n0[SR], n1[SR];
+    // register prefetch of the next staged level: raw bits of q0 and the
+    // FP32 q1 (unless it is q0), used only after the next barrier
+    uint32_t n0r[SR];
+    float n1[SR];
 #pragma unroll
     for (int r = 0; r < SR; r++) {
-        n0[r] = n1[r] = 0.0f;
+        n0r[r] = 0u;
+        n1[r] = 0.0f;
         if (row_in[r]) {
             const unsigned c = rowbase[r] + sk * (unsigned)m0;
-            n0[r] = ld_q(p.q0, c, p.kind0, p.scale0);
-            n1[r] = same ? n0[r] : q1p[c];
+            n0r[r] = ld_raw(p.q0, c, p.kind0);
+            if (!same) n1[r] = q1p[c];
         }
     }
 
@@ -535,12 +574,14 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
         const int L1 = m - 1, L2 = m - 2;
         // ---------------- (A) stage level m, prefetch level m+1 and the
         //                  coefficients of this iteration
+        float n0[SR];
 #pragma unroll
         for (int r = 0; r < SR; r++) {
             const int row = ly + 8 * r;
+            n0[r] = cvt_q(n0r[r], p.kind0, p.scale0);
             if (m < nz && r < nsr) {
                 sq0[m % 3][row][lx] = n0[r];
-                sq1[m & 3][row][lx] = n1[r];
+                sq1[m & 3][row][lx] = same ? n0[r] : n1[r];
             }
         }
 #pragma unroll
@@ -555,8 +596,8 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
             for (int r = 0; r < SR; r++) {
                 if (row_in[r]) {
                     const unsigned c = rowbase[r] + sk * (unsigned)(m + 1);
-                    n0[r] = ld_q(p.q0, c, p.kind0, p.scale0);
-                    n1[r] = same ? n0[r] : q1p[c];
+                    n0r[r] = ld_raw(p.q0, c, p.kind0);
+                    if (!same) n1[r] = q1p[c];
                 }
             }
         }
