@@ -1178,7 +1178,9 @@ std::map<std::string, std::pair<int, int>> g_tuned;
 
 std::string tune_key(const FastBatch *b)
 {
-    std::string k = std::to_string(b->order) + "/" + std::to_string(b->fct);
+    // the 16-bit and FP32-only batches run different kernel instantiations
+    std::string k = std::to_string(b->order) + "/" + std::to_string(b->fct) + "/" +
+                    std::to_string((int)b->q16);
     for (const auto &dm : b->doms)
         k += ":" + std::to_string(dm.st->d.nx) + "x" + std::to_string(dm.st->d.ny) + "x" +
              std::to_string(dm.st->d.nz) + "s" + std::to_string(dm.st->sp.size());
