@@ -526,7 +526,7 @@ int icar_adv_bind_species(void *handle, int idx, float *host, int storage, int p
         s.dev = nullptr;
     }
     if (!s.dev) {
-        rc = dalloc(h, &s.dev, h->N * elem);
+        rc = dalloc(h, &s.dev, h->N * elem + 64);  // the fused kernels read 16-bit data in aligned pairs
         if (rc) return rc;
     }
     if (storage != ICAR_ADV_STORE_F32 && !h->stage) {

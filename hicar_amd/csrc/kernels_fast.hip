@@ -74,17 +74,35 @@ __device__ __forceinline__ bool is_bf16(int kind)
 // load makes the wave wait for it.
 typedef __attribute__((address_space(1))) const uint32_t gcu32;
 
+// One 32-bit load for either width: FP32 element i, or the aligned pair of
+// 16-bit elements holding element i (16-bit buffers are padded so that the
+// pair of the last element is in bounds).  The same instruction for every
+// storage kind, so a batch mixing kinds needs no branch around its loads.
 __device__ __forceinline__ uint32_t ld_raw(const void *p, size_t i, int kind)
 {
-    return kind == ICAR_ADV_STORE_F32 ? ((gcu32 *)p)[i] : (uint32_t)((gcu16 *)p)[i];
+    return ((gcu32 *)p)[kind == ICAR_ADV_STORE_F32 ? i : (i >> 1)];
 }
 
-__device__ __forceinline__ float cvt_q(uint32_t raw, int kind, float scale)
+// the value of element i from the word ld_raw returned for it
+// the same with a branch on the kind: fewer live registers where the kind is
+// the same for the whole block
+__device__ __forceinline__ float cvt_q_uniform(uint32_t w, size_t i, int kind, float scale)
 {
-    if (kind == ICAR_ADV_STORE_F32) return __uint_as_float(raw);
+    if (kind == ICAR_ADV_STORE_F32) return __uint_as_float(w);
+    const uint32_t raw = (i & 1) ? (w >> 16) : (w & 0xFFFFu);
     const float v = is_bf16(kind) ? __uint_as_float(raw << 16)
                                   : __half2float(__ushort_as_half((uint16_t)raw));
     return v * scale;
+}
+
+// (branch-free: both 16-bit forms are decoded and the kind selects)
+__device__ __forceinline__ float cvt_q(uint32_t w, size_t i, int kind, float scale)
+{
+    const uint32_t raw = (i & 1) ? (w >> 16) : (w & 0xFFFFu);
+    const float vb = __uint_as_float(raw << 16);
+    const float vh = __half2float(__ushort_as_half((uint16_t)raw));
+    const float v16 = (is_bf16(kind) ? vb : vh) * scale;
+    return kind == ICAR_ADV_STORE_F32 ? __uint_as_float(w) : v16;
 }
 
 __device__ __forceinline__ float ld_q(const void *p, size_t i, int kind, float scale)
@@ -417,13 +435,13 @@ __global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ 
                 kind = sp[s0 + b].kind0;
                 sc = sp[s0 + b].scale0;
             }
-            qc[b] = cvt_q(rc[b], kind, sc);
-            qw[b] = cvt_q(rw[b], kind, sc);
-            qe[b] = cvt_q(re[b], kind, sc);
-            qs[b] = cvt_q(rs[b], kind, sc);
-            qn[b] = cvt_q(rn[b], kind, sc);
-            qu[b] = cvt_q(ru[b], kind, sc);
-            qd[b] = cvt_q(rd[b], kind, sc);
+            qc[b] = cvt_q(rc[b], c, kind, sc);
+            qw[b] = cvt_q(rw[b], c - 1, kind, sc);
+            qe[b] = cvt_q(re[b], c + 1, kind, sc);
+            qs[b] = cvt_q(rs[b], c - sj, kind, sc);
+            qn[b] = cvt_q(rn[b], c + sj, kind, sc);
+            qu[b] = cvt_q(ru[b], c + sk, kind, sc);
+            qd[b] = cvt_q(rd[b], c - sk, kind, sc);
         }
 #pragma unroll
         for (int b = 0; b < FSB; b++) {
@@ -578,7 +596,7 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
 #pragma unroll
         for (int r = 0; r < SR; r++) {
             const int row = ly + 8 * r;
-            n0[r] = cvt_q(n0r[r], p.kind0, p.scale0);
+            n0[r] = cvt_q_uniform(n0r[r], rowbase[r] + sk * (unsigned)m, p.kind0, p.scale0);
             if (m < nz && r < nsr) {
                 sq0[m % 3][row][lx] = n0[r];
                 sq1[m & 3][row][lx] = same ? n0[r] : n1[r];
@@ -1083,7 +1101,7 @@ int fast_set_species(FastState *st, const FastSpecies *sp, int n)
     if (n == 0) return 0;
     st->alt.assign(n, nullptr);
     for (int s = 0; s < n; s++)
-        if (hipMalloc(&st->alt[s], st->N * kind_bytes(sp[s].storage)) != hipSuccess) {
+        if (hipMalloc(&st->alt[s], st->N * kind_bytes(sp[s].storage) + 64) != hipSuccess) {
             st->err = "hipMalloc of a species buffer failed";
             return ICAR_ADV_ENOMEM;
         }
