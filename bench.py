@@ -108,6 +108,38 @@ def measure_ensemble(steps: int, warmup: int, n: int = 48):
     return out
 
 
+def measure_wind_steps(case, reps: int = 10):
+    """balance_uvw and compute_dt (the two wind steps next to advect) on the
+    device copies of the workload's winds: HIP-event time per launch and
+    algorithmic GB/s (inputs read once, w written once)."""
+    import numpy as np
+
+    from hicar_amd import synthetic
+    from hicar_amd.advection import MpdataAdvection
+    case.options.kernel_variant = 1
+    st = MpdataAdvection(case.domain, case.options)
+    out = {}
+    try:
+        dzl = synthetic.dz_levels(case.domain.nz).astype(np.float32)
+        st.balance_w()
+        st.compute_dt(0.9, 3, dzl)
+        st.sync()
+        st.profile(True)
+        for _ in range(reps):
+            st.balance_w()
+            st.compute_dt(0.9, 3, dzl)
+        stats = st.kernel_stats()
+        st.profile(False)
+        for name in ("wind_balance", "dt_bound"):
+            k = stats[name]
+            ms = k["total_ms"] / k["launches"]
+            gbs = k["alg_bytes"] / k["total_ms"] / 1e6
+            out[name] = {"avg_ms": ms, "achieved_GBps": gbs, "hbm_frac": gbs * 1e9 / HBM_PEAK}
+    finally:
+        st.close()
+    return out
+
+
 def set_bf16(case):
     from hicar_amd import _lib, synthetic
     d = case.domain
@@ -222,6 +254,11 @@ def main() -> int:
                          "b_step": step_bytes(d), "launch_config": lc}
         log(f"  {name}: {results[name]['value'] / 1e9:.2f} G cell-species/s, "
             f"{1e3 * t / args.steps:.2f} ms/step")
+    wind = None
+    if args.variant == "all":
+        wind = measure_wind_steps(case)
+        log("  wind steps: " + ", ".join(f"{k} {v['avg_ms']:.3f} ms ({v['achieved_GBps']:.0f} GB/s)"
+                                         for k, v in wind.items()))
     # launch-bound small domain: eager vs graph replay
     small = {}
     if args.variant == "all" and not args.no_small:
@@ -298,6 +335,7 @@ def main() -> int:
         "compulsory_hbm_fraction": b_step * args.steps / r["seconds"] / HBM_PEAK,
         "launch_bound_256x64x20": small,
         "ensemble_48": ensemble,
+        "wind_steps": wind,
     }
     if not args.no_cpu:
         log("CPU baseline (oracle, 1 thread) ...")

@@ -39,7 +39,8 @@ EXPORTS = (
     "icar_adv_ensemble_destroy", "icar_adv_ensemble_run", "icar_adv_ensemble_sync",
     "icar_adv_ensemble_last_error", "icar_adv_ensemble_profile", "icar_adv_ensemble_kernel_stats",
     "icar_adv_launch_config", "icar_adv_ensemble_launch_config", "icar_adv_balance_w",
-    "icar_adv_compute_dt",
+    "icar_adv_compute_dt", "icar_adv_set_relax_filter", "icar_adv_set_forcing",
+    "icar_adv_apply_forcing",
 )
 
 
@@ -106,6 +107,9 @@ def load() -> ctypes.CDLL:
         "icar_adv_launch_config": ([vp, i3, i3], ip),
         "icar_adv_balance_w": ([vp], ip),
         "icar_adv_compute_dt": ([vp, ctypes.c_float, ip, fp, fp], ip),
+        "icar_adv_set_relax_filter": ([vp, ip, ip, ip, ip, ip], ip),
+        "icar_adv_set_forcing": ([vp, ip, fp, fp], ip),
+        "icar_adv_apply_forcing": ([vp, ctypes.c_double], ip),
         "icar_adv_ensemble_launch_config": ([vp, i3, i3], ip),
     }
     for name, (args, res) in sig.items():

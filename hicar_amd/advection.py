@@ -124,6 +124,26 @@ class MpdataAdvection:
                                                  _ptr(dzl), ctypes.byref(dt)))
         return dt.value
 
+    def set_relax_filter(self, west: bool, east: bool, north: bool, south: bool,
+                         halo: int = 2) -> None:
+        """init_relax_filters (domain_obj.f90:3285-3390) for the domain's
+        boundary flags and halo width."""
+        self._check(self.lib.icar_adv_set_relax_filter(self.h, int(west), int(east), int(north),
+                                                        int(south), int(halo)))
+
+    def set_forcing(self, name: str, forcing_hi: np.ndarray, dqdt: np.ndarray) -> None:
+        """Uploads a species' boundary forcing and its tendency per second."""
+        f = np.ascontiguousarray(forcing_hi, dtype=np.float32)
+        t = np.ascontiguousarray(dqdt, dtype=np.float32)
+        if f.shape != self.domain.shape or t.shape != self.domain.shape:
+            raise ValueError("forcing arrays need the mass-grid shape")
+        self._check(self.lib.icar_adv_set_forcing(self.h, self.names.index(name), _ptr(f),
+                                                   _ptr(t)))
+
+    def apply_forcing(self, dt: float) -> None:
+        """apply_forcing (domain_obj.f90:3493-3503) of every forced species."""
+        self._check(self.lib.icar_adv_apply_forcing(self.h, ctypes.c_double(dt)))
+
     def set_graph(self, enable: bool) -> None:
         """Record the run() loop in a HIP graph and replay it."""
         self._check(self.lib.icar_adv_set_graph(self.h, int(enable)))

@@ -32,6 +32,7 @@ struct SpeciesSlot {
     int pow2 = 0;
     void *dev = nullptr;  // float*, or uint16_t* for BF16/FP16 storage
     bool bound = false;
+    float *fhi = nullptr, *dqdt = nullptr;  // lateral forcing (icar_adv_set_forcing)
 };
 
 struct TimedLaunch {
@@ -60,6 +61,7 @@ struct Handle {
     double *d_partial = nullptr;
     std::vector<double> h_partial;
     float *d_dt = nullptr;  // dz_levels (nz) then the dt partial maxima (4 per block)
+    float *d_filter = nullptr;  // relaxation filter (nx * ny), icar_adv_set_relax_filter
     std::string err;
     FastState *fast = nullptr;
     FastBatch *batch = nullptr;  // this domain as a batch of one (fused path)
@@ -196,7 +198,12 @@ void free_all(Handle *h)
     float *bufs[] = {h->sf.U, h->sf.V, h->sf.W, h->sf.G, h->sf.Gz, h->sf.Wd, h->q2, h->u2,
                      h->v2,   h->w2,   h->u2l, h->v2l, h->w2l, h->stage};
     for (float *p : bufs) (void)hipFree(p);
-    for (auto &s : h->sp) (void)hipFree(s.dev);
+    for (auto &s : h->sp) {
+        (void)hipFree(s.dev);
+        (void)hipFree(s.fhi);
+        (void)hipFree(s.dqdt);
+    }
+    (void)hipFree(h->d_filter);
     (void)hipFree(h->d_partial);
     (void)hipFree(h->d_dt);
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
@@ -677,8 +684,11 @@ int icar_adv_balance_w(void *handle)
     const int i_s = std::max(g.its - 1, g.ims) - g.ims, i_e = std::min(g.ite + 1, g.ime) - g.ims;
     const int j_s = std::max(g.jts - 1, g.jms) - g.jms, j_e = std::min(g.jte + 1, g.jme) - g.jms;
     DomainFields f = domain_fields(h);
-    launch_balance_w(h->stream, h->d, i_s, i_e, j_s, j_e, g.dx, h->o.advect_density, f,
-                     h->dev_field[ICAR_ADV_FIELD_W]);
+    // reads u, jacobian_u, v, jacobian_v, density, dz, jacobian_w; writes w
+    timed(h, K_BALANCE, 4.0 * (2.0 * h->Nu + 2.0 * h->Nv + 4.0 * h->N), [&] {
+        launch_balance_w(h->stream, h->d, i_s, i_e, j_s, j_e, g.dx, h->o.advect_density, f,
+                         h->dev_field[ICAR_ADV_FIELD_W]);
+    });
     HIPCHK(h, hipGetLastError());
     if (h->o.debug) HIPCHK(h, hipStreamSynchronize(h->stream));
     return 0;
@@ -707,7 +717,8 @@ int icar_adv_compute_dt(void *handle, float cfl, int cfl_strictness, const float
     float *part = h->d_dt + d.nz;
     HIPCHK(h, hipMemcpyAsync(h->d_dt, dz_levels, sizeof(float) * d.nz, hipMemcpyHostToDevice,
                              h->stream));
-    timed(h, K_REDUCE, 4.0 * 2 * cells(h), [&] {
+    // reads u, v and w once (the tile's second read hits the caches)
+    timed(h, K_DT, 4.0 * ((double)h->Nu + (double)h->Nv + (double)h->N), [&] {
         launch_dt_partials(h->stream, d, its, ite, jts, jte, g.dx,
                            h->dev_field[ICAR_ADV_FIELD_U], h->dev_field[ICAR_ADV_FIELD_V],
                            h->dev_field[ICAR_ADV_FIELD_W], h->d_dt, part);
@@ -744,6 +755,155 @@ int icar_adv_compute_dt(void *handle, float cfl, int cfl_strictness, const float
         return set_err(h, ICAR_ADV_ECOURANT,
                        "time step too small: dt = %g (Umax = %g, Vmax = %g, Wmax = %g)",
                        (double)*dt, (double)mu, (double)mv, (double)mw);
+    return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// init_relax_filters, src/objects/domain_obj.f90:3285-3390: the 2-D
+// relaxation filter f[j*nx + i] for a domain with halo hs and the given
+// boundary flags, statement by statement (later statements overwrite
+// earlier ones; a domain touching boundaries in x and y takes the corner
+// branches, of which only the first matching pair applies, so a one-image
+// domain relaxes its north and west edges).
+void relax_filter(int nx, int ny, int hs, bool west, bool east, bool north, bool south,
+                  std::vector<float> &f)
+{
+    static const float rs[7] = {0.9f, 0.75f, 0.6f, 0.5f, 0.4f, 0.25f, 0.1f};
+    static const float rs_r[7] = {0.1f, 0.25f, 0.4f, 0.5f, 0.6f, 0.75f, 0.9f};
+    f.assign((size_t)nx * ny, 0.0f);
+    auto at = [&](int i, int j) -> float & { return f[(size_t)j * nx + i]; };
+    const int nr = std::min(7, std::min(nx - 1 - hs, ny - 1 - hs));  // :3302
+    auto cols = [&](int i0, int i1, int j0, int j1) {  // value 1 on [i0,i1) x [j0,j1)
+        for (int j = std::max(j0, 0); j < std::min(j1, ny); j++)
+            for (int i = std::max(i0, 0); i < std::min(i1, nx); i++) at(i, j) = 1.0f;
+    };
+    auto ramp_x = [&](int i0, const float *r, int j0, int j1) {  // at(i0+t, j) = r[t]
+        for (int j = std::max(j0, 0); j < std::min(j1, ny); j++)
+            for (int t = 0; t < nr; t++)
+                if (i0 + t >= 0 && i0 + t < nx) at(i0 + t, j) = r[t];
+    };
+    auto ramp_y = [&](int j0, const float *r, int i0, int i1) {  // at(i, j0+t) = r[t]
+        for (int i = std::max(i0, 0); i < std::min(i1, nx); i++)
+            for (int t = 0; t < nr; t++)
+                if (j0 + t >= 0 && j0 + t < ny) at(i, j0 + t) = r[t];
+    };
+    auto square = [&](bool xlo, bool ylo) {  // the corner square, rs(min(a, b))
+        for (int a = 1; a <= nr; a++)
+            for (int b = 1; b <= nr; b++) {
+                const int i = xlo ? hs + a - 1 : nx - hs - a;
+                const int j = ylo ? hs + b - 1 : ny - hs - b;
+                if (i >= 0 && i < nx && j >= 0 && j < ny) at(i, j) = rs[std::min(a, b) - 1];
+            }
+    };
+    const bool corner = (west || east) && (north || south);
+    if (!corner) {
+        if (west) {
+            cols(0, hs, 0, ny);
+            ramp_x(hs, rs, 0, ny);
+        } else if (east) {
+            cols(nx - hs, nx, 0, ny);
+            ramp_x(nx - hs - nr, rs_r, 0, ny);
+        } else if (north) {
+            cols(0, nx, ny - hs - 2, ny);  // rows jme-hs-1:jme (:3320)
+            ramp_y(ny - hs - nr, rs_r, 0, nx);
+        } else if (south) {
+            cols(0, nx, 0, hs);
+            ramp_y(hs, rs, 0, nx);
+        }
+    } else if (north && west) {
+        cols(0, hs, 0, ny);
+        cols(0, nx, ny - hs - 2, ny);
+        ramp_x(hs, rs, 0, ny - hs);
+        ramp_y(ny - hs - nr, rs_r, hs, nx);
+        square(true, false);
+    } else if (north && east) {
+        cols(nx - hs, nx, 0, ny);
+        cols(0, nx, ny - hs - 2, ny);
+        ramp_x(nx - hs - nr, rs_r, 0, ny - hs);
+        ramp_y(ny - hs - nr, rs_r, 0, nx - hs);
+        square(false, false);
+    } else if (south && west) {
+        cols(0, hs, 0, ny);
+        cols(0, nx, 0, hs);
+        ramp_x(hs, rs, hs, ny);
+        ramp_y(hs, rs, hs, nx);
+        square(true, true);
+    } else if (south && east) {
+        cols(nx - hs, nx, 0, ny);
+        cols(0, nx, 0, hs);
+        ramp_x(nx - hs - nr, rs_r, hs, ny);
+        ramp_y(hs, rs, 0, nx - hs);
+        square(false, true);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int icar_adv_set_relax_filter(void *handle, int west, int east, int north, int south, int halo)
+{
+    Handle *h;
+    int rc = check_handle(handle, &h);
+    if (rc) return rc;
+    if (halo < 0 || halo >= std::min(h->d.nx, h->d.ny) - 1)
+        return set_err(h, ICAR_ADV_ERANGE, "halo %d out of range", halo);
+    std::vector<float> f;
+    relax_filter(h->d.nx, h->d.ny, halo, west != 0, east != 0, north != 0, south != 0, f);
+    if (!h->d_filter) {
+        rc = dalloc(h, (void **)&h->d_filter, sizeof(float) * f.size());
+        if (rc) return rc;
+    }
+    HIPCHK(h, hipMemcpyAsync(h->d_filter, f.data(), sizeof(float) * f.size(),
+                             hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int icar_adv_set_forcing(void *handle, int idx, const float *forcing_hi, const float *dqdt)
+{
+    Handle *h;
+    int rc = check_handle(handle, &h);
+    if (rc) return rc;
+    if (idx < 0 || idx >= h->nspecies || !h->sp[idx].bound)
+        return set_err(h, ICAR_ADV_ERANGE, "species %d is not bound", idx);
+    if (!forcing_hi || !dqdt) return set_err(h, ICAR_ADV_EINVAL, "NULL forcing array");
+    SpeciesSlot &s = h->sp[idx];
+    if (s.storage != ICAR_ADV_STORE_F32)
+        return set_err(h, ICAR_ADV_EINVAL, "lateral forcing needs an FP32-stored species");
+    if (!s.fhi) {
+        rc = dalloc(h, (void **)&s.fhi, sizeof(float) * h->N);
+        if (!rc) rc = dalloc(h, (void **)&s.dqdt, sizeof(float) * h->N);
+        if (rc) return rc;
+    }
+    HIPCHK(h, hipMemcpyAsync(s.fhi, forcing_hi, sizeof(float) * h->N, hipMemcpyHostToDevice,
+                             h->stream));
+    HIPCHK(h, hipMemcpyAsync(s.dqdt, dqdt, sizeof(float) * h->N, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int icar_adv_apply_forcing(void *handle, double dt)
+{
+    Handle *h;
+    int rc = check_handle(handle, &h);
+    if (rc) return rc;
+    if (!h->d_filter)
+        return set_err(h, ICAR_ADV_EUNBOUND, "icar_adv_set_relax_filter has not been called");
+    for (int s = 0; s < h->nspecies; s++) {
+        SpeciesSlot &sl = h->sp[s];
+        if (!sl.bound || !sl.fhi) continue;
+        // reads the forcing, its tendency and the species, writes both fields
+        timed(h, K_FORCING, 5.0 * 4 * cells(h), [&] {
+            launch_forcing(h->stream, h->d, dt, h->d_filter, static_cast<float *>(species_dev(h, s)),
+                           sl.fhi, sl.dqdt);
+        });
+        HIPCHK(h, hipGetLastError());
+    }
+    if (h->o.debug) HIPCHK(h, hipStreamSynchronize(h->stream));
     return 0;
 }
 
@@ -1130,7 +1290,8 @@ const char *kernel_name(int id)
     static const char *names[K_COUNT] = {"courant",     "upwind",     "pseudo_velocity",
                                          "fct",         "ring_bc",    "reduce",
                                          "copy",        "fused_first_pass", "fused_corrective_pass",
-                                         "convert16",   "face_coefficients"};
+                                         "convert16",   "face_coefficients", "wind_balance",
+                                         "dt_bound",    "forcing"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "";
 }
 }  // namespace icaradv

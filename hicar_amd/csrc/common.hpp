@@ -36,6 +36,9 @@ enum KernelId {
     K_FUSED_CORR,
     K_CONVERT,
     K_COEF,
+    K_BALANCE,
+    K_DT,
+    K_FORCING,
     K_COUNT
 };
 
@@ -72,6 +75,8 @@ int mass_partials_blocks(Dims d);
 void launch_balance_w(hipStream_t s, Dims d, int i_s, int i_e, int j_s, int j_e, float dx,
                       int adv_den, const DomainFields &f, float *w);
 int dt_partial_blocks();
+void launch_forcing(hipStream_t s, Dims d, double dt, const float *filt, float *q, float *fhi,
+                    const float *dqdt);
 void launch_dt_partials(hipStream_t s, Dims d, int its, int ite, int jts, int jte, float dx,
                         const float *u, const float *v, const float *w, const float *dzl,
                         float *part);

@@ -1,5 +1,5 @@
-// The two wind-field steps ICAR runs next to advect, on the device copies of
-// the winds the advection handle already holds:
+// The steps ICAR runs next to advect, on the device copies of the fields the
+// advection handle already holds:
 //   k_balance_w   balance_uvw (src/physics/wind.f90:88-164, update=.false.):
 //                 the horizontal mass-flux divergence (calc_divergence with
 //                 horz_only, :201-234) and w integrated up each column so
@@ -12,6 +12,8 @@
 //                 rate; the host takes the maximum over blocks and finishes
 //                 in FP32 as the reference does (a maximum is exact in any
 //                 order, so the result is the reference's bits).
+//   k_forcing     the boundary-forced branch of apply_forcing
+//                 (src/objects/domain_obj.f90:3493-3503) for one species.
 // Every expression keeps the Fortran's evaluation order; this unit is
 // compiled with -ffp-contract=off.
 #include "common.hpp"
@@ -171,6 +173,47 @@ void launch_dt_partials(hipStream_t s, Dims d, int its, int ite, int jts, int jt
 {
     hipLaunchKernelGGL(k_dt_partials, dim3(DT_BLOCKS), dim3(DT_THREADS), 0, s, d, its, ite, jts,
                        jte, dx, u, v, w, dzl, part);
+}
+
+}  // namespace icaradv
+
+namespace icaradv {
+
+namespace {
+
+// apply_forcing for a boundary-forced 3-D variable (domain_obj.f90:3493-3503):
+// the forcing advances by its tendency, then the value takes it where the
+// filter is 1 and relaxes towards it where the filter is below 1.  dt is
+// double precision in the reference (dt%seconds()), so the tendency and the
+// relaxation term are formed in FP64 and rounded to FP32 on assignment, as
+// the Fortran does; forcing - value is an FP32 difference.
+__global__ void __launch_bounds__(256)
+    k_forcing(Dims d, double dt, const float *__restrict__ filt, float *__restrict__ q,
+              float *__restrict__ fhi, const float *__restrict__ dqdt)
+{
+    const size_t n = (size_t)d.nx * d.nz * d.ny;
+    for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n; c += (size_t)gridDim.x * 256) {
+        const float f = (float)((double)fhi[c] + ((double)dqdt[c] * dt));
+        fhi[c] = f;
+        const size_t i = c % (size_t)d.nx, j = c / ((size_t)d.nx * d.nz);
+        const float rf = filt[j * (size_t)d.nx + i];
+        if (rf == 1.0f)
+            q[c] = f;
+        else if (rf < 1.0f)
+            q[c] = (float)((double)q[c] + ((double)rf * dt / 3600.0) * (double)(f - q[c]));
+    }
+}
+
+}  // namespace
+
+void launch_forcing(hipStream_t s, Dims d, double dt, const float *filt, float *q, float *fhi,
+                    const float *dqdt)
+{
+    const size_t n = (size_t)d.nx * d.nz * d.ny;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_forcing, dim3((unsigned)blocks), dim3(256), 0, s, d, dt, filt, q, fhi,
+                       dqdt);
 }
 
 }  // namespace icaradv

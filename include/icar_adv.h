@@ -174,6 +174,25 @@ int icar_adv_balance_w(void *handle);
 int icar_adv_compute_dt(void *handle, float cfl, int cfl_strictness, const float *dz_levels,
                         float *dt);
 
+/* Lateral forcing of advected species, the step that fills the outer cells
+ * in ICAR's time step before the physics and advect (apply_forcing,
+ * src/objects/domain_obj.f90:3457-3530, called at src/main/time_step.f90:319),
+ * on the device copies.
+ *
+ * icar_adv_set_relax_filter = init_relax_filters (domain_obj.f90:3285-3390):
+ * the relaxation filter for the domain's boundary flags (non-zero = the edge
+ * is a domain boundary) and halo width (grid%halo_size).
+ * icar_adv_set_forcing uploads species idx's forcing field (forcing_hi
+ * data_3d) and its tendency per second (dqdt_3d), both on the mass grid, as
+ * update_delta_fields sets them (:3405-3445); FP32-stored species only.
+ * icar_adv_apply_forcing advances every forced species over dt seconds:
+ * forcing += dqdt*dt everywhere; where the filter is 1 the species takes the
+ * forcing, where it is below 1 it relaxes by filter*dt/3600 of the
+ * difference (double precision intermediates, as dt%seconds() is). */
+int icar_adv_set_relax_filter(void *handle, int west, int east, int north, int south, int halo);
+int icar_adv_set_forcing(void *handle, int idx, const float *forcing_hi, const float *dqdt);
+int icar_adv_apply_forcing(void *handle, double dt);
+
 /* Message of the last failure on this handle (handle may be NULL). */
 const char *icar_adv_last_error(void *handle);
 

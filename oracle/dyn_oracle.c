@@ -1,6 +1,6 @@
 /*
- * CPU restatement of the two steps ICAR runs next to advect on the wind
- * field: the wind balance and the time-step bound.
+ * CPU restatement of the steps ICAR runs next to advect: the wind balance,
+ * the time-step bound and the lateral forcing of advected species.
  *
  * TEST INFRASTRUCTURE ONLY, like mpdata_oracle.c (same build, same real
  * type): the HIP library is checked against it and never links it.
@@ -12,6 +12,9 @@
  *   compute_dt, src/main/time_step.f90:51-168, with use_density=.false. as
  *   update_dt passes it (:229-242): its density branches are commented out
  *   in the reference and are not restated.
+ *   init_relax_filters, src/objects/domain_obj.f90:3285-3390, and the 3-D
+ *   boundary-forced branch of apply_forcing, :3493-3503 (called at
+ *   src/main/time_step.f90:319 before the physics and advect).
  *
  * Layout as mpdata_oracle.c: IX(i,k,j) = i + nx*(k + nz*j); u and jacobian_u
  * are (nx+1, nz, ny), v and jacobian_v (nx, nz, ny+1).  Every expression is
@@ -194,4 +197,116 @@ real oracle_compute_dt(int nx, int ny, int nz, int its, int ite, int jts, int jt
         }
     }
     return cfl / maxwind3d;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Lateral forcing of advected species (the step that fills the ring and     */
+/* relaxes the border before advect in ICAR's time step)                     */
+/* ------------------------------------------------------------------------- */
+
+/* init_relax_filters, src/objects/domain_obj.f90:3285-3390: the 2-D filter
+ * f[j*nx + i] (0-based) of a domain with halo hs and the given boundary
+ * flags, statement by statement (later statements overwrite earlier ones;
+ * a domain touching boundaries in both x and y takes the corner branches,
+ * of which only the first matching pair applies). */
+void oracle_relax_filter(int nx, int ny, int hs, int west, int east, int north, int south,
+                         real *f)
+{
+    static const real rs[7] = {L(0.9), L(0.75), L(0.6), L(0.5), L(0.4), L(0.25), L(0.1)};
+    static const real rs_r[7] = {L(0.1), L(0.25), L(0.4), L(0.5), L(0.6), L(0.75), L(0.9)};
+#define F(i, j) f[(size_t)(j) * nx + (i)]
+    int nr = 7;
+    if (nx - 1 - hs < nr) nr = nx - 1 - hs; /* :3302 min(7, ime-ims-hs, jme-jms-hs) */
+    if (ny - 1 - hs < nr) nr = ny - 1 - hs;
+    for (size_t c = 0; c < (size_t)nx * ny; c++) f[c] = 0;
+    const int corner = (west || east) && (north || south);
+    if (!corner) {
+        if (west) { /* :3309-3313 */
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < hs; i++) F(i, j) = 1;
+            for (int j = 0; j < ny; j++)
+                for (int r = 0; r < nr; r++) F(hs + r, j) = rs[r];
+        } else if (east) { /* :3314-3318 */
+            for (int j = 0; j < ny; j++)
+                for (int i = nx - hs; i < nx; i++) F(i, j) = 1;
+            for (int j = 0; j < ny; j++)
+                for (int r = 0; r < nr; r++) F(nx - hs - nr + r, j) = rs_r[r];
+        } else if (north) { /* :3319-3323, rows jme-hs-1:jme */
+            for (int j = ny - hs - 2; j < ny; j++)
+                for (int i = 0; i < nx; i++) F(i, j) = 1;
+            for (int i = 0; i < nx; i++)
+                for (int r = 0; r < nr; r++) F(i, ny - hs - nr + r) = rs_r[r];
+        } else if (south) { /* :3324-3328 */
+            for (int j = 0; j < hs; j++)
+                for (int i = 0; i < nx; i++) F(i, j) = 1;
+            for (int i = 0; i < nx; i++)
+                for (int r = 0; r < nr; r++) F(i, hs + r) = rs[r];
+        }
+    } else if (north && west) { /* :3331-3345 */
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < hs; i++) F(i, j) = 1;
+        for (int j = ny - hs - 2; j < ny; j++)
+            for (int i = 0; i < nx; i++) F(i, j) = 1;
+        for (int j = 0; j <= ny - 1 - hs; j++)
+            for (int r = 0; r < nr; r++) F(hs + r, j) = rs[r];
+        for (int i = hs; i < nx; i++)
+            for (int r = 0; r < nr; r++) F(i, ny - hs - nr + r) = rs_r[r];
+        for (int a = 1; a <= nr; a++)
+            for (int b = 1; b <= nr; b++) F(hs + a - 1, ny - hs - b) = rs[(a < b ? a : b) - 1];
+    } else if (north && east) { /* :3346-3360 */
+        for (int j = 0; j < ny; j++)
+            for (int i = nx - hs; i < nx; i++) F(i, j) = 1;
+        for (int j = ny - hs - 2; j < ny; j++)
+            for (int i = 0; i < nx; i++) F(i, j) = 1;
+        for (int j = 0; j <= ny - 1 - hs; j++)
+            for (int r = 0; r < nr; r++) F(nx - hs - nr + r, j) = rs_r[r];
+        for (int i = 0; i <= nx - 1 - hs; i++)
+            for (int r = 0; r < nr; r++) F(i, ny - hs - nr + r) = rs_r[r];
+        for (int a = 1; a <= nr; a++)
+            for (int b = 1; b <= nr; b++) F(nx - hs - a, ny - hs - b) = rs[(a < b ? a : b) - 1];
+    } else if (south && west) { /* :3361-3375 */
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < hs; i++) F(i, j) = 1;
+        for (int j = 0; j < hs; j++)
+            for (int i = 0; i < nx; i++) F(i, j) = 1;
+        for (int j = hs; j < ny; j++)
+            for (int r = 0; r < nr; r++) F(hs + r, j) = rs[r];
+        for (int i = hs; i < nx; i++)
+            for (int r = 0; r < nr; r++) F(i, hs + r) = rs[r];
+        for (int a = 1; a <= nr; a++)
+            for (int b = 1; b <= nr; b++) F(hs + a - 1, hs + b - 1) = rs[(a < b ? a : b) - 1];
+    } else if (south && east) { /* :3376-3390 */
+        for (int j = 0; j < ny; j++)
+            for (int i = nx - hs; i < nx; i++) F(i, j) = 1;
+        for (int j = 0; j < hs; j++)
+            for (int i = 0; i < nx; i++) F(i, j) = 1;
+        for (int j = hs; j < ny; j++)
+            for (int r = 0; r < nr; r++) F(nx - hs - nr + r, j) = rs_r[r];
+        for (int i = 0; i <= nx - 1 - hs; i++)
+            for (int r = 0; r < nr; r++) F(i, hs + r) = rs[r];
+        for (int a = 1; a <= nr; a++)
+            for (int b = 1; b <= nr; b++) F(nx - hs - a, hs + b - 1) = rs[(a < b ? a : b) - 1];
+    }
+#undef F
+}
+
+/* apply_forcing for a 3-D variable forced on the boundaries,
+ * domain_obj.f90:3493-3503.  dt%seconds() is double precision, so the
+ * tendency and the relaxation term are formed in double and the sums
+ * rounded to the real kind on assignment; forcing - value is a difference
+ * of two reals. */
+void oracle_apply_forcing(int nx, int ny, int nz, double dt, const real *filt, real *q,
+                          real *fhi, const real *dqdt)
+{
+    for (int j = 0; j < ny; j++)
+        for (int k = 0; k < nz; k++)
+            for (int i = 0; i < nx; i++) {
+                const size_t c = IX(i, k, j);
+                fhi[c] = (real)((double)fhi[c] + ((double)dqdt[c] * dt));
+                const real rf = filt[(size_t)j * nx + i];
+                if (rf == L(1.0))
+                    q[c] = fhi[c];
+                else if (rf < L(1.0))
+                    q[c] = (real)((double)q[c] + ((double)rf * dt / 3600.0) * (double)(fhi[c] - q[c]));
+            }
 }

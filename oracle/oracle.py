@@ -177,6 +177,25 @@ class Oracle:
                                           self.real(dom.dx), self._p(u), self._p(v), self._p(w),
                                           self._p(dzl), self.real(cfl), int(strictness))
 
+    # -- lateral forcing (dyn_oracle.c) ------------------------------------
+    def relax_filter(self, nx, ny, hs, west, east, north, south):
+        """init_relax_filters (domain_obj.f90:3285-3390): the (ny, nx) filter."""
+        f = np.zeros((ny, nx), self.dtype)
+        self.lib.oracle_relax_filter(nx, ny, hs, int(west), int(east), int(north), int(south),
+                                     self._p(f))
+        return f
+
+    def apply_forcing(self, q, fhi, dqdt, filt, dt):
+        """apply_forcing (domain_obj.f90:3493-3503) for one 3-D boundary-forced
+        variable: returns the new (q, forcing_hi)."""
+        nx, ny, nz = self._dims(q)
+        q, fhi = self.arr(q).copy(), self.arr(fhi).copy()
+        dqdt, filt = self.arr(dqdt), self.arr(filt)
+        assert filt.shape == (ny, nx)
+        self.lib.oracle_apply_forcing(nx, ny, nz, ctypes.c_double(dt), self._p(filt), self._p(q),
+                                      self._p(fhi), self._p(dqdt))
+        return q, fhi
+
     def divergence(self, U, V, W, dz):
         nx, ny, nz = self._dims(U)
         mx = ctypes.c_double()

@@ -92,3 +92,56 @@ def test_host_mirror_balance_then_dt_then_advect(oracle32):
             np.testing.assert_array_equal(d.species[n], ref[n], err_msg=n)
     finally:
         st.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("flags", [(1, 1, 1, 1), (1, 0, 0, 0), (0, 1, 0, 1), (0, 0, 1, 0)])
+def test_lateral_forcing_matches_oracle(oracle32, variant, flags):
+    """init_relax_filters + apply_forcing on the device copies: two forced
+    species and one unforced, two forcing updates, bit for bit the oracle's
+    (and the species the fused path keeps in its alternate buffers too)."""
+    case = windy_case(14, nx=40, ny=33, nz=7)
+    d, o = case.domain, case.options
+    r = np.random.default_rng(15)
+    o.vars_to_advect = ["qv", "qc", "qtr"]
+    d.set_var_lists(o, {n: r.uniform(0.5, 2.0, d.shape) for n in o.vars_to_advect})
+    o.kernel_variant = variant
+    forcing = {n: (r.uniform(0.5, 2.0, d.shape).astype(np.float32),
+                   r.uniform(-1e-4, 1e-4, d.shape).astype(np.float32)) for n in ("qv", "qtr")}
+    filt = oracle32.relax_filter(d.nx, d.ny, 2, *flags)
+    ref = {n: a.copy() for n, a in d.species.items()}
+    fhi = {n: f.copy() for n, (f, _) in forcing.items()}
+    st = MpdataAdvection(d, o)
+    try:
+        if variant == 1:  # the fused path keeps alternating home buffers
+            st.advect(case.dt)
+            st.download()
+            ref = {n: a.copy() for n, a in d.species.items()}
+        st.set_relax_filter(*flags, halo=2)
+        for n, (f, t) in forcing.items():
+            st.set_forcing(n, f, t)
+        for dt in (37.5, 12.25):
+            st.apply_forcing(dt)
+            for n, (_, t) in forcing.items():
+                ref[n], fhi[n] = oracle32.apply_forcing(ref[n], fhi[n], t, filt, dt)
+        st.download()
+    finally:
+        st.close()
+    for n in ref:
+        np.testing.assert_array_equal(d.species[n], ref[n], err_msg=n)
+    assert not np.array_equal(ref["qv"], ref["qc"])
+
+
+def test_forcing_needs_fp32_storage():
+    case = windy_case(16, nx=20, ny=12, nz=5)
+    d, o = case.domain, case.options
+    o.kernel_variant = 1
+    d.storage["qv"] = (_lib.STORE_BF16, 0)
+    st = MpdataAdvection(d, o)
+    try:
+        with pytest.raises(_lib.IcarAdvError):
+            st.set_forcing("qv", d.species["qv"], np.zeros(d.shape, np.float32))
+        with pytest.raises(_lib.IcarAdvError):  # no filter yet
+            st.apply_forcing(10.0)
+    finally:
+        st.close()

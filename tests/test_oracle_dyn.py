@@ -1,5 +1,5 @@
-"""Pins the CPU restatement of the wind balance and the time-step bound
-(oracle/dyn_oracle.c) to what the mathematics fixes:
+"""Pins the CPU restatement of the wind balance, the time-step bound and the
+lateral forcing (oracle/dyn_oracle.c) to what the mathematics fixes:
 
   calc_divergence   closed forms for linear winds and a linearly varying
                     density, including the one-sided density at the domain
@@ -13,6 +13,12 @@
                     and an independent FP64 NumPy evaluation of the tile
                     maximum (the w of the level below, zoffset) for varying
                     winds
+  relax filter      the ramp values and widths of each edge; mirror and
+                    transpose symmetries between the edges and corners; the
+                    north edge's two extra rows; the single-image domain
+                    relaxing only its north and west edges
+  apply_forcing     closed forms of the forcing advance and of the
+                    relaxation at filter values 1, 0.5, 0.25 and 0
 Every test runs on the FP32 and the FP64 build.
 """
 import numpy as np
@@ -158,3 +164,82 @@ def test_compute_dt_tile_maximum(orc):
     assert orc.compute_dt(d, dzl, 0.9, 3, its, ite, jts, jte) == got
     assert orc.compute_dt(d, dzl, 0.9, 1, its, ite, jts, jte) == pytest.approx(
         0.9 / (500.0 * np.sqrt(3.0) * 1.001), rel=1e-6)
+
+
+# --------------------------------------------------------------------------
+# init_relax_filters (domain_obj.f90:3285-3390)
+# --------------------------------------------------------------------------
+RS = np.array([0.9, 0.75, 0.6, 0.5, 0.4, 0.25, 0.1])
+
+
+def test_relax_filter_single_edges(orc):
+    """One boundary: hs columns (rows) of 1 at the edge, then the ramp
+    0.9 ... 0.1 inward over nr = min(7, n-1-hs) cells, 0 beyond; east and
+    north are the mirror images of west and south, south the transpose of
+    west."""
+    nx, ny, hs = 16, 13, 2
+    west = orc.relax_filter(nx, ny, hs, 1, 0, 0, 0)
+    exp_row = np.zeros(nx)
+    exp_row[:hs] = 1
+    exp_row[hs:hs + 7] = RS
+    np.testing.assert_allclose(west, np.broadcast_to(exp_row, (ny, nx)), rtol=1e-7)
+    east = orc.relax_filter(nx, ny, hs, 0, 1, 0, 0)
+    np.testing.assert_array_equal(east, west[:, ::-1])
+    south = orc.relax_filter(ny, nx, hs, 0, 0, 0, 1)  # (nx rows, ny columns)
+    np.testing.assert_array_equal(south, orc.relax_filter(nx, ny, hs, 1, 0, 0, 0).T)
+    north = orc.relax_filter(ny, nx, hs, 0, 0, 1, 0)
+    np.testing.assert_array_equal(north, south[::-1])
+
+
+def test_relax_filter_north_sets_two_extra_rows(orc):
+    """The north edge sets hs + 2 rows to 1 (jme-hs-1:jme, :3320) before the
+    ramp overwrites the inner ones; with a one-cell ramp (nr = 1) one of them
+    survives, three rows from the edge."""
+    nx, ny, hs = 8, 4, 2        # nr = min(7, 5, 1) = 1
+    f = orc.relax_filter(nx, ny, hs, 0, 0, 1, 0)
+    np.testing.assert_allclose(f[:, 0], [1.0, 0.1, 1.0, 1.0], rtol=1e-7)
+
+
+def test_relax_filter_corners(orc):
+    """A domain touching x and y boundaries takes the corner branches: the
+    single-image domain (all four) relaxes only its north and west edges;
+    the corner square holds the ramp value of the nearer edge,
+    rs(min(a, b)); south-west is north-west mirrored in y."""
+    nx, ny, hs = 14, 12, 2
+    f = orc.relax_filter(nx, ny, hs, 1, 1, 1, 1)
+    np.testing.assert_array_equal(f, orc.relax_filter(nx, ny, hs, 1, 0, 1, 0))
+    assert np.all(f[:, :hs] == 1) and np.all(f[ny - hs:] == 1)
+    for a in range(1, 8):
+        for b in range(1, 8):
+            assert f[ny - hs - b, hs + a - 1] == pytest.approx(RS[min(a, b) - 1], rel=1e-7)
+    # nothing at the east and south edges
+    assert np.all(f[:ny - hs - 7, hs + 7:] == 0)
+    sw = orc.relax_filter(nx, ny, hs, 1, 0, 0, 1)
+    np.testing.assert_array_equal(sw, f[::-1])
+    ne = orc.relax_filter(nx, ny, hs, 0, 1, 1, 0)
+    np.testing.assert_array_equal(ne, f[:, ::-1])
+
+
+# --------------------------------------------------------------------------
+# apply_forcing (domain_obj.f90:3493-3503)
+# --------------------------------------------------------------------------
+def test_apply_forcing_closed_forms(orc):
+    """forcing_hi advances by dqdt*dt everywhere; where the filter is 1 the
+    value becomes it, where it is below 1 the value relaxes by
+    filter*dt/3600 of the difference (filter 0: unchanged)."""
+    nx, ny, nz = 5, 4, 3
+    shape = (ny, nz, nx)
+    q = orc.arr(np.full(shape, 1.0))
+    fhi = orc.arr(np.full(shape, 1.5))
+    dqdt = orc.arr(np.full(shape, 1.0 / 7200.0))
+    filt = orc.arr(np.zeros((ny, nx)))
+    filt[:, 0] = 1.0
+    filt[:, 1] = 0.5
+    filt[:, 2] = 0.25
+    dt = 3600.0
+    qn, fn = orc.apply_forcing(q, fhi, dqdt, filt, dt)
+    np.testing.assert_allclose(fn, 2.0, rtol=1e-7)             # 1.5 + dt/7200
+    np.testing.assert_allclose(qn[:, :, 0], 2.0, rtol=1e-7)     # = forcing
+    np.testing.assert_allclose(qn[:, :, 1], 1.5, rtol=1e-7)     # 1 + 0.5*(2-1)
+    np.testing.assert_allclose(qn[:, :, 2], 1.25, rtol=1e-7)    # 1 + 0.25*(2-1)
+    np.testing.assert_array_equal(qn[:, :, 3:], q[:, :, 3:])    # filter 0
