@@ -109,9 +109,10 @@ def measure_ensemble(steps: int, warmup: int, n: int = 48):
 
 
 def measure_wind_steps(case, reps: int = 10):
-    """balance_uvw and compute_dt (the two wind steps next to advect) on the
-    device copies of the workload's winds: HIP-event time per launch and
-    algorithmic GB/s (inputs read once, w written once)."""
+    """The steps next to advect on the device copies: balance_uvw and
+    compute_dt on the workload's winds, and apply_forcing of every species
+    with the one-image relaxation filter: HIP-event time per launch and
+    algorithmic GB/s (inputs read once, outputs written once)."""
     import numpy as np
 
     from hicar_amd import synthetic
@@ -121,16 +122,23 @@ def measure_wind_steps(case, reps: int = 10):
     out = {}
     try:
         dzl = synthetic.dz_levels(case.domain.nz).astype(np.float32)
+        st.set_relax_filter(True, True, True, True, halo=2)
+        zero = np.zeros(case.domain.shape, np.float32)
+        for n in st.names:
+            if case.domain.storage.get(n, (0, 0))[0] == 0:
+                st.set_forcing(n, case.domain.species[n], zero)
         st.balance_w()
         st.compute_dt(0.9, 3, dzl)
+        st.apply_forcing(10.0)
         st.sync()
         st.profile(True)
         for _ in range(reps):
             st.balance_w()
             st.compute_dt(0.9, 3, dzl)
+            st.apply_forcing(10.0)
         stats = st.kernel_stats()
         st.profile(False)
-        for name in ("wind_balance", "dt_bound"):
+        for name in ("wind_balance", "dt_bound", "forcing"):
             k = stats[name]
             ms = k["total_ms"] / k["launches"]
             gbs = k["alg_bytes"] / k["total_ms"] / 1e6

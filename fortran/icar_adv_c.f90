@@ -18,6 +18,7 @@ module icar_adv_c
     public :: icar_adv_max_courant, icar_adv_last_error, icar_adv_sync, icar_adv_apply_bc
     public :: icar_adv_run, icar_adv_set_graph, icar_adv_error_message
     public :: icar_adv_balance_w, icar_adv_compute_dt, icar_adv_launch_config
+    public :: icar_adv_set_relax_filter, icar_adv_set_forcing, icar_adv_apply_forcing
     public :: ICAR_ADV_FIELD_U, ICAR_ADV_FIELD_V, ICAR_ADV_FIELD_W, ICAR_ADV_FIELD_DENSITY
     public :: ICAR_ADV_FIELD_JACOBIAN, ICAR_ADV_FIELD_JACOBIAN_U, ICAR_ADV_FIELD_JACOBIAN_V
     public :: ICAR_ADV_FIELD_JACOBIAN_W, ICAR_ADV_FIELD_ADVECTION_DZ
@@ -163,6 +164,30 @@ module icar_adv_c
             integer(c_int), value :: cfl_strictness
             real(c_float), intent(in) :: dz_levels(*)
             real(c_float), intent(out) :: dt
+        end function
+
+        !> init_relax_filters (domain_obj.f90:3285-3390) for the boundary flags
+        integer(c_int) function icar_adv_set_relax_filter(handle, west, east, north, south, halo) &
+                bind(C, name="icar_adv_set_relax_filter")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+            integer(c_int), value :: west, east, north, south, halo
+        end function
+
+        !> a species' forcing_hi%data_3d and dqdt_3d (update_delta_fields)
+        integer(c_int) function icar_adv_set_forcing(handle, idx, forcing_hi, dqdt) &
+                bind(C, name="icar_adv_set_forcing")
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: handle
+            integer(c_int), value :: idx
+            real(c_float), intent(in) :: forcing_hi(*), dqdt(*)
+        end function
+
+        !> apply_forcing (domain_obj.f90:3493-3503) of the forced species
+        integer(c_int) function icar_adv_apply_forcing(handle, dt) bind(C, name="icar_adv_apply_forcing")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: handle
+            real(c_double), value :: dt
         end function
 
         integer(c_int) function icar_adv_launch_config(handle, corr_cfg, ksplit) &
