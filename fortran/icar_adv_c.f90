@@ -26,7 +26,7 @@ module icar_adv_c
     public :: ICAR_ADV_STORE_BF16_SR, ICAR_ADV_STORE_F16_SR
     public :: ICAR_ADV_MASK_ALL_FIELDS, ICAR_ADV_MASK_ALL_SPECIES
     public :: ICAR_ADV_OK, ICAR_ADV_EINVAL, ICAR_ADV_ESHAPE, ICAR_ADV_EUNBOUND, ICAR_ADV_ERANGE
-    public :: ICAR_ADV_ENOMEM, ICAR_ADV_ECOURANT
+    public :: ICAR_ADV_ENOMEM, ICAR_ADV_ECOURANT, ICAR_ADV_STD, ICAR_ADV_MPDATA
 
     integer(c_int), parameter :: ICAR_ADV_FIELD_U = 0, ICAR_ADV_FIELD_V = 1, ICAR_ADV_FIELD_W = 2
     integer(c_int), parameter :: ICAR_ADV_FIELD_DENSITY = 3, ICAR_ADV_FIELD_JACOBIAN = 4
@@ -37,6 +37,7 @@ module icar_adv_c
     integer(c_int), parameter :: ICAR_ADV_OK = 0, ICAR_ADV_EINVAL = -1, ICAR_ADV_ESHAPE = -2
     integer(c_int), parameter :: ICAR_ADV_EUNBOUND = -3, ICAR_ADV_ERANGE = -4, ICAR_ADV_ENOMEM = -5
     integer(c_int), parameter :: ICAR_ADV_ECOURANT = -6
+    integer(c_int), parameter :: ICAR_ADV_STD = 1, ICAR_ADV_MPDATA = 2  ! kADV_STD, kADV_MPDATA
     integer(c_long_long), parameter :: ICAR_ADV_MASK_ALL_FIELDS = int(z'1FF', c_long_long)
     ! bits 16..63 (species 0..47); written as a negative two's-complement value
     integer(c_long_long), parameter :: ICAR_ADV_MASK_ALL_SPECIES = -65536_c_long_long
@@ -49,7 +50,9 @@ module icar_adv_c
     end type
 
     !> namelist switches: mpdata_order, MPDATA_FCT, advect_density, debug, RK3
-    !! (options_obj.f90:1568,1579-1584,1025,2198) and the kernel variant
+    !! (options_obj.f90:1568,1579-1584,1025,2198), the kernel variant, and
+    !! for the standard scheme adv, h_order, v_order, flux_corr
+    !! (options_obj.f90:470-490,1582-1584)
     type, bind(C) :: icar_adv_opts
         integer(c_int) :: mpdata_order = 2
         integer(c_int) :: mpdata_fct = 1
@@ -57,6 +60,10 @@ module icar_adv_c
         integer(c_int) :: debug = 0
         integer(c_int) :: rk3 = 0
         integer(c_int) :: kernel_variant = 1
+        integer(c_int) :: advection = 2
+        integer(c_int) :: h_order = 1
+        integer(c_int) :: v_order = 1
+        integer(c_int) :: flux_corr = 0
     end type
 
     interface

@@ -23,7 +23,8 @@ STORE_NAMES = {0: "f32", 1: "bf16", 2: "f16", 3: "bf16-sr", 4: "f16-sr"}
 MAX_SPECIES = 48
 MASK_ALL_FIELDS = 0x1FF
 MASK_ALL_SPECIES = 0xFFFFFFFFFFFF0000
-ECOURANT = -6
+EINVAL, ESHAPE, EUNBOUND, ERANGE, ENOMEM, ECOURANT = -1, -2, -3, -4, -5, -6
+ADV_STD, ADV_MPDATA = 1, 2
 
 
 def mask_species(s: int) -> int:
@@ -53,7 +54,7 @@ class Grid(ctypes.Structure):
 class Opts(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in
                 ("mpdata_order", "mpdata_fct", "advect_density", "debug", "rk3",
-                 "kernel_variant")]
+                 "kernel_variant", "advection", "h_order", "v_order", "flux_corr")]
 
 
 class IcarAdvError(RuntimeError):

@@ -1,4 +1,4 @@
-"""Host-side mirror of ICAR's advection module for the MPDATA scheme.
+"""Host-side mirror of ICAR's advection module (MPDATA and the standard scheme).
 
 ``adv_var_request``, ``adv_init`` and ``advect`` keep the names, arguments and
 behaviour of ``src/physics/advection_driver.f90:23-126``; the work happens on
@@ -22,8 +22,10 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(_fp)
 
 
-class MpdataAdvection:
-    """One device-resident MPDATA state for one domain (one C-ABI handle)."""
+class Advection:
+    """One device-resident advection state for one domain (one C-ABI
+    handle): MPDATA (adv=2) or the standard scheme (adv=1), as the options
+    select."""
 
     def __init__(self, domain: Domain, options: Options):
         domain.check()
@@ -32,9 +34,12 @@ class MpdataAdvection:
         self.options = options
         g = _lib.Grid(domain.ims, domain.ime, domain.kms, domain.kme, domain.jms, domain.jme,
                       domain.its, domain.ite, domain.jts, domain.jte, domain.dx)
-        o = _lib.Opts(int(options.adv_options.mpdata_order), int(options.adv_options.MPDATA_FCT),
+        ao = options.adv_options
+        o = _lib.Opts(int(ao.mpdata_order), int(ao.MPDATA_FCT),
                       int(options.parameters.advect_density), int(options.parameters.debug),
-                      int(options.time_options.RK3), int(options.kernel_variant))
+                      int(options.time_options.RK3), int(options.kernel_variant),
+                      int(options.physics.advection), int(ao.h_order), int(ao.v_order),
+                      int(ao.flux_corr))
         h = ctypes.c_void_p()
         _lib.check(self.lib.icar_adv_create(ctypes.byref(g), ctypes.byref(o), ctypes.byref(h)))
         self.h = h
@@ -172,7 +177,7 @@ class MpdataAdvection:
 
 
 class Ensemble:
-    """Independent domains (one MpdataAdvection each, kernel_variant 1)
+    """Independent MPDATA domains (one Advection each, kernel_variant 1)
     advanced together: one launch per pass over every (domain, tile,
     species), the step loop optionally recorded in a HIP graph."""
 
@@ -228,26 +233,28 @@ class Ensemble:
             pass
 
 
+# the name of the MPDATA-only first version of this class
+MpdataAdvection = Advection
+
+
 def adv_var_request(options: Options) -> None:
     """advection_driver.f90:46-57 -> adv_std_var_request (advect.f90:94-108):
     both schemes advect water vapour and potential temperature."""
     options.advect_vars(["qv", "theta"])
 
 
-def adv_init(domain: Domain, options: Options) -> MpdataAdvection | None:
-    """advection_driver.f90:26-44.  For MPDATA, creates the device state
-    (mpdata_init) and uploads the domain; returns it and keeps it on the
+def adv_init(domain: Domain, options: Options) -> Advection:
+    """advection_driver.f90:26-44: creates the device state (mpdata_init or
+    adv_std_init) and uploads the domain; returns it and keeps it on the
     domain for ``advect``."""
-    if options.physics.advection == kADV_MPDATA:
-        domain._mpdata = MpdataAdvection(domain, options)
+    if options.physics.advection in (kADV_MPDATA, kADV_STD):
+        domain._mpdata = Advection(domain, options)
         return domain._mpdata
-    if options.physics.advection == kADV_STD:
-        raise NotImplementedError("adv=1 (standard advection) is not part of this library")
     raise ValueError(f"unknown advection scheme {options.physics.advection}")
 
 
 def balance_uvw(domain: Domain, options: Options) -> None:
-    """wind.f90:88-164 (update=.false.) for a domain whose MPDATA state
+    """wind.f90:88-164 (update=.false.) for a domain whose device state
     adv_init created: w is rebalanced on the device from the device u, v,
     density and metric (time_step.f90:324 runs it before advect when
     advect_density is set)."""
@@ -268,12 +275,14 @@ def compute_dt(domain: Domain, options: Options, dz_levels) -> float:
 
 
 def advect(domain: Domain, options: Options, dt: float, sync_host: bool = True) -> None:
-    """advection_driver.f90:59-126 for adv=2: Courant numbers once, then every
-    advected species in registry order, on the device.  Under RK3 MPDATA does
-    nothing (:112-115).  With ``sync_host`` the host species arrays are
-    updated in place on return, as the reference's are."""
-    if options.physics.advection != kADV_MPDATA:
-        raise NotImplementedError("only adv=2 (MPDATA) is provided by this library")
+    """advection_driver.f90:59-126: the winds once (mpdata_compute_wind or
+    adv_std_compute_wind), then every advected species in registry order, on
+    the device.  adv=1 takes the three RK3 calls or one Euler call of
+    adv_std_advect3d; under RK3 MPDATA does nothing (:112-115).  With
+    ``sync_host`` the host species arrays are updated in place on return, as
+    the reference's are."""
+    if options.physics.advection not in (kADV_MPDATA, kADV_STD):
+        raise ValueError(f"unknown advection scheme {options.physics.advection}")
     st = getattr(domain, "_mpdata", None)
     if st is None:
         raise RuntimeError("adv_init(domain, options) has not been called")

@@ -32,6 +32,7 @@ UNITS = [
     ("kernels_exact.hip", ["-ffp-contract=off"]),
     ("kernels_fast.hip", ["-ffp-contract=off"]),
     ("kernels_dyn.hip", ["-ffp-contract=off"]),
+    ("kernels_std.hip", ["-ffp-contract=off"]),
     ("api.hip", ["-ffp-contract=off"]),
 ]
 

@@ -1,11 +1,15 @@
-// C-ABI of the MPDATA advection library: handle management, host<->device
+// C-ABI of the advection library: handle management, host<->device
 // residency, the advect() orchestration and diagnostics.
 //
-// advect() follows src/physics/advection_driver.f90:59-126 for adv=2:
+// advect() follows src/physics/advection_driver.f90:59-126.  For adv=2:
 // Courant numbers once (mpdata_compute_wind, adv_mpdata.f90:477-504), then
 // each bound species through the order loop of mpdata_advect3d
-// (adv_mpdata.f90:352-413).  Species are independent inside advect, so the
-// fused variant advances all of them per launch with identical results.
+// (adv_mpdata.f90:352-413).  For adv=1: the mass fluxes once
+// (adv_std_compute_wind, advect.f90:444-492), then each species by the three
+// RK3 calls of adv_std_advect3d (advection_driver.f90:98-110) or one Euler
+// call (:117-118), the last call with the WRF limiter when flux_corr is 1.
+// Species are independent inside advect, so the batched kernels advance
+// many of them per launch with identical results.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -70,6 +74,16 @@ struct Handle {
     hipGraphExec_t gexec = nullptr;
     float g_dt = 0.0f;
     int g_steps = -1, g_bcx = -1, g_bcy = -1, g_par0 = -1, g_par1 = -1;
+    // standard scheme (adv=1): mass fluxes, divisors, and per species the
+    // buffer the next state is written to (swapped with the species' own
+    // after each step: std_par counts the swaps mod 2), the second RK3 stage
+    // and the limiter's dumb_q / scale_in / scale_out
+    bool is_std = false;
+    float *Um = nullptr, *Vm = nullptr, *Wm = nullptr, *Rj = nullptr, *Rdj = nullptr;
+    float *std_alt[ICAR_ADV_MAX_SPECIES] = {}, *std_b[ICAR_ADV_MAX_SPECIES] = {};
+    float *std_dq[ICAR_ADV_MAX_SPECIES] = {}, *std_si[ICAR_ADV_MAX_SPECIES] = {};
+    float *std_so[ICAR_ADV_MAX_SPECIES] = {};
+    int std_par = 0;
     // profiling
     bool prof = false;
     std::vector<TimedLaunch> pending;
@@ -203,6 +217,12 @@ void free_all(Handle *h)
         (void)hipFree(s.fhi);
         (void)hipFree(s.dqdt);
     }
+    float *sbufs[] = {h->Um, h->Vm, h->Wm, h->Rj, h->Rdj};
+    for (float *p : sbufs) (void)hipFree(p);
+    for (int s = 0; s < ICAR_ADV_MAX_SPECIES; s++) {
+        float *per[] = {h->std_alt[s], h->std_b[s], h->std_dq[s], h->std_si[s], h->std_so[s]};
+        for (float *p : per) (void)hipFree(p);
+    }
     (void)hipFree(h->d_filter);
     (void)hipFree(h->d_partial);
     (void)hipFree(h->d_dt);
@@ -332,12 +352,125 @@ int divergence_check(Handle *h)
     return 0;
 }
 
+// adv=1: the standard scheme for every bound species, in batches of up to
+// STD_MAX_BATCH species per launch.
+int advect_std(Handle *h, float dt)
+{
+    const DomainFields f = domain_fields(h);
+    const double n = cells(h);
+    timed(h, K_STD_WIND, 14.0 * 4 * n, [&] {  // 9 inputs, 5 outputs
+        launch_std_wind(h->stream, h->d, h->g.dx, dt, h->o.advect_density, f, h->Um, h->Vm,
+                        h->Wm, h->Rj, h->Rdj);
+    });
+    StdGeom g;
+    g.d = h->d;
+    g.its = h->g.its - h->g.ims;
+    g.ite = h->g.ite - h->g.ims;
+    g.jts = h->g.jts - h->g.jms;
+    g.jte = h->g.jte - h->g.jms;
+    g.horder = h->o.h_order;
+    g.vorder = h->o.v_order;
+    g.Um = h->Um;
+    g.Vm = h->Vm;
+    g.Wm = h->Wm;
+    g.Rj = h->Rj;
+    g.Rdj = h->Rdj;
+    const bool fc = h->o.flux_corr == 1, rk3 = h->o.rk3 != 0;
+    std::vector<int> idx;
+    for (int s = 0; s < h->nspecies; s++)
+        if (h->sp[s].bound) idx.push_back(s);
+    for (size_t b0 = 0; b0 < idx.size(); b0 += STD_MAX_BATCH) {
+        StdBatch b{};
+        b.n = (int)std::min(idx.size() - b0, (size_t)STD_MAX_BATCH);
+        const double ns = (double)b.n;
+        auto fill = [&](int which_in) {
+            for (int m = 0; m < b.n; m++) {
+                const int s = idx[b0 + m];
+                float *home = static_cast<float *>(h->sp[s].dev);
+                b.qold[m] = home;
+                b.qin[m] = which_in == 0 ? home : which_in == 1 ? h->std_alt[s] : h->std_b[s];
+                b.dq[m] = h->std_dq[s];
+                b.sin[m] = h->std_si[s];
+                b.sout[m] = h->std_so[s];
+            }
+        };
+        auto outs = [&](int which_out) {
+            for (int m = 0; m < b.n; m++) {
+                const int s = idx[b0 + m];
+                b.out[m] = which_out == 1 ? h->std_alt[s] : h->std_b[s];
+            }
+        };
+        // algorithmic bytes: winds and divisors once, then per species the
+        // arrays the kernel reads or writes once each
+        const double wb = 5.0 * 4 * n;
+        if (rk3) {  // advection_driver.f90:103-110
+            fill(0);
+            outs(1);
+            timed(h, K_STD_STAGE, wb + ns * (fc ? 3 : 2) * 4 * n,
+                  [&] { launch_std_stage(h->stream, g, 0.333f, fc ? 1 : 0, b); });
+            fill(1);
+            outs(2);
+            timed(h, K_STD_STAGE, wb + ns * 3 * 4 * n,
+                  [&] { launch_std_stage(h->stream, g, 0.5f, 0, b); });
+            fill(2);
+            outs(1);
+        } else {  // :118, the species is both qfluxes and qold
+            fill(0);
+            outs(1);
+            if (fc)
+                timed(h, K_STD_STAGE, wb + ns * 2 * 4 * n,
+                      [&] { launch_std_stage(h->stream, g, 1.0f, 2, b); });
+        }
+        if (fc) {
+            timed(h, K_STD_SCALE, wb + ns * 5 * 4 * n, [&] { launch_std_scale(h->stream, g, b); });
+            timed(h, K_STD_FINAL, wb + ns * 6 * 4 * n, [&] { launch_std_final(h->stream, g, b); });
+        } else {
+            timed(h, K_STD_STAGE, wb + ns * (rk3 ? 3 : 2) * 4 * n,
+                  [&] { launch_std_stage(h->stream, g, 1.0f, 0, b); });
+        }
+    }
+    HIPCHK(h, hipGetLastError());
+    // the new states are in the alternate buffers: make them the species'
+    for (int s : idx) {
+        void *cur = h->sp[s].dev;
+        h->sp[s].dev = h->std_alt[s];
+        h->std_alt[s] = static_cast<float *>(cur);
+    }
+    h->std_par ^= 1;
+    return 0;
+}
+
+// Which of its two buffers each species lives in: the fused MPDATA path and
+// the standard scheme alternate buffers every step.  A recorded graph holds
+// the buffer addresses, so it replays only from the parity it was recorded at.
+int parity(Handle *h)
+{
+    if (h->fast) return fast_parity(h->fast);
+    return h->is_std ? h->std_par : 0;
+}
+
+void set_parity(Handle *h, int p)
+{
+    if (h->fast) {
+        fast_set_parity(h->fast, p);
+    } else if (h->is_std && h->std_par != p) {
+        for (int s = 0; s < h->nspecies; s++) {
+            if (!h->sp[s].bound) continue;
+            void *cur = h->sp[s].dev;
+            h->sp[s].dev = h->std_alt[s];
+            h->std_alt[s] = static_cast<float *>(cur);
+        }
+        h->std_par = p;
+    }
+}
+
 int do_advect(Handle *h, float dt)
 {
     int rc = require_bound(h);
     if (rc) return rc;
     h->last_dt = dt;
     h->have_dt = true;
+    if (h->is_std) return advect_std(h, dt);
     const DomainFields f = domain_fields(h);
     const double n = cells(h);
     if (h->o.kernel_variant == 1 && !h->o.rk3 && h->nspecies > 0 && h->o.mpdata_order >= 1) {
@@ -411,12 +544,38 @@ int apply_bc_on(Handle *h, T *owner, hipStream_t stream, int bc_x, int bc_y)
     return 0;
 }
 
+// Options and bounds the standard scheme needs: orders 1/3/5, flux_corr 0/1,
+// and a halo wide enough for flux3's stencil (plus one cell for the limiter's
+// extended tile), as ICAR's grid gives it (domain_obj.f90:2979-2989).
+int std_check(const icar_adv_grid *g, const icar_adv_opts *o)
+{
+    auto odd = [](int v) { return v == 1 || v == 3 || v == 5; };
+    if (!odd(o->h_order) || !odd(o->v_order))
+        return set_err(nullptr, ICAR_ADV_EINVAL, "h_order %d / v_order %d: only 1, 3 and 5",
+                       o->h_order, o->v_order);
+    if (o->flux_corr != 0 && o->flux_corr != 1)
+        return set_err(nullptr, ICAR_ADV_EINVAL, "flux_corr %d unknown (0 none, 1 WRF monotonic)",
+                       o->flux_corr);
+    const int need = (o->h_order + 1) / 2 + (o->flux_corr == 1 ? 1 : 0);
+    const int hw = std::min(g->its - g->ims, g->ime - g->ite);
+    const int hs = std::min(g->jts - g->jms, g->jme - g->jte);
+    if (g->ite < g->its || g->jte < g->jts || hw < need || hs < need)
+        return set_err(nullptr, ICAR_ADV_ESHAPE,
+                       "h_order %d%s needs a halo of %d cells around the tile (have %d in x, %d "
+                       "in y)",
+                       o->h_order, o->flux_corr == 1 ? " with the limiter" : "", need, hw, hs);
+    if (o->v_order == 5 && g->kme - g->kms + 1 < 4)
+        return set_err(nullptr, ICAR_ADV_ESHAPE, "v_order 5 needs at least 4 levels");
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
 
 int icar_adv_create(const icar_adv_grid *grid, const icar_adv_opts *opts, void **handle)
 {
+    int rc_create = 0;
     if (!grid || !opts || !handle) return set_err(nullptr, ICAR_ADV_EINVAL, "NULL argument");
     *handle = nullptr;
     if (!grid_ok(grid))
@@ -429,7 +588,16 @@ int icar_adv_create(const icar_adv_grid *grid, const icar_adv_opts *opts, void *
                        opts->mpdata_order);
     if (opts->kernel_variant < 0 || opts->kernel_variant > 1)
         return set_err(nullptr, ICAR_ADV_EINVAL, "kernel_variant %d unknown", opts->kernel_variant);
+    const int adv = opts->advection == 0 ? ICAR_ADV_MPDATA : opts->advection;
+    if (adv != ICAR_ADV_STD && adv != ICAR_ADV_MPDATA)
+        return set_err(nullptr, ICAR_ADV_EINVAL, "advection %d unknown (1 standard, 2 MPDATA)",
+                       opts->advection);
+    if (adv == ICAR_ADV_STD) {
+        rc_create = std_check(grid, opts);
+        if (rc_create) return rc_create;
+    }
     Handle *h = new Handle();
+    h->is_std = adv == ICAR_ADV_STD;
     h->g = *grid;
     h->o = *opts;
     h->d = {grid->ime - grid->ims + 1, grid->jme - grid->jms + 1, grid->kme - grid->kms + 1};
@@ -449,7 +617,16 @@ int icar_adv_create(const icar_adv_grid *grid, const icar_adv_opts *opts, void *
             free_all(h);
             return set_err(nullptr, ICAR_ADV_ENOMEM, "%s", m.c_str());
         }
-    if (h->o.kernel_variant == 0) {
+    if (h->is_std) {  // mass fluxes and divisors of the standard scheme
+        const size_t sz[] = {h->Nu * sizeof(float), h->Nv * sizeof(float), fb, fb, fb};
+        float **sb[] = {&h->Um, &h->Vm, &h->Wm, &h->Rj, &h->Rdj};
+        for (int q = 0; q < 5; q++)
+            if (dalloc(h, (void **)sb[q], sz[q])) {
+                std::string m = h->err;
+                free_all(h);
+                return set_err(nullptr, ICAR_ADV_ENOMEM, "%s", m.c_str());
+            }
+    } else if (h->o.kernel_variant == 0) {
         float **work[] = {&h->q2, &h->u2, &h->v2, &h->w2, &h->u2l, &h->v2l, &h->w2l};
         for (float **p : work)
             if (dalloc(h, (void **)p, fb)) {
@@ -520,6 +697,8 @@ int icar_adv_bind_species(void *handle, int idx, float *host, int storage, int p
     if (!host) return set_err(h, ICAR_ADV_EINVAL, "NULL array for species %d", idx);
     if (storage < ICAR_ADV_STORE_F32 || storage > ICAR_ADV_STORE_F16_SR)
         return set_err(h, ICAR_ADV_EINVAL, "unknown storage %d", storage);
+    if (storage != ICAR_ADV_STORE_F32 && h->is_std)
+        return set_err(h, ICAR_ADV_EINVAL, "the standard scheme keeps FP32 storage");
     if (storage != ICAR_ADV_STORE_F32 && h->o.kernel_variant == 0)
         return set_err(h, ICAR_ADV_EINVAL,
                        "reduced-precision storage needs kernel_variant 1 (the exact kernels are "
@@ -539,6 +718,17 @@ int icar_adv_bind_species(void *handle, int idx, float *host, int storage, int p
     if (storage != ICAR_ADV_STORE_F32 && !h->stage) {
         rc = dalloc(h, (void **)&h->stage, h->N * sizeof(float));
         if (rc) return rc;
+    }
+    if (h->is_std) {  // next-state buffer, second RK3 stage, limiter work arrays
+        float **per[] = {&h->std_alt[idx], &h->std_b[idx], &h->std_dq[idx], &h->std_si[idx],
+                         &h->std_so[idx]};
+        const bool need[] = {true, h->o.rk3 != 0, h->o.flux_corr == 1, h->o.flux_corr == 1,
+                             h->o.flux_corr == 1};
+        for (int q = 0; q < 5; q++)
+            if (need[q] && !*per[q]) {
+                rc = dalloc(h, (void **)per[q], h->N * sizeof(float) + 64);
+                if (rc) return rc;
+            }
     }
     s.host = host;
     s.storage = storage;
@@ -950,10 +1140,10 @@ int icar_adv_run(void *handle, float dt, int nsteps, int bc_x, int bc_y)
     if (!h->use_graph || h->prof || h->o.debug || nsteps == 0) return steps();
     const bool fused = h->fast && h->o.kernel_variant == 1 && !h->o.rk3 && h->nspecies > 0 &&
                        h->o.mpdata_order >= 1;
-    const int p0 = h->fast ? fast_parity(h->fast) : 0;
+    const int p0 = parity(h);
     if (h->gexec && h->g_dt == dt && h->g_steps == nsteps && h->g_bcx == bc_x &&
         h->g_bcy == bc_y && h->g_par0 == p0) {
-        if (h->fast) fast_set_parity(h->fast, h->g_par1);
+        set_parity(h, h->g_par1);
         HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
         return 0;
     }
@@ -976,7 +1166,7 @@ int icar_adv_run(void *handle, float dt, int nsteps, int bc_x, int bc_y)
     hipError_t ec = hipStreamEndCapture(h->stream, &g);
     if (rc || ec != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
-        if (h->fast) fast_set_parity(h->fast, p0);
+        set_parity(h, p0);
         if (rc) return rc;
         return set_err(h, (int)ec, "hipStreamEndCapture failed: %s", hipGetErrorString(ec));
     }
@@ -984,7 +1174,7 @@ int icar_adv_run(void *handle, float dt, int nsteps, int bc_x, int bc_y)
     (void)hipGraphDestroy(g);
     if (ec != hipSuccess) {
         h->gexec = nullptr;
-        if (h->fast) fast_set_parity(h->fast, p0);
+        set_parity(h, p0);
         return set_err(h, (int)ec, "hipGraphInstantiate failed: %s", hipGetErrorString(ec));
     }
     h->g_dt = dt;
@@ -992,7 +1182,7 @@ int icar_adv_run(void *handle, float dt, int nsteps, int bc_x, int bc_y)
     h->g_bcx = bc_x;
     h->g_bcy = bc_y;
     h->g_par0 = p0;
-    h->g_par1 = h->fast ? fast_parity(h->fast) : 0;  // advanced while recording
+    h->g_par1 = parity(h);  // advanced while recording
     HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
     return 0;
 }
@@ -1143,12 +1333,12 @@ int icar_adv_ensemble_create(int n, void *const *handles, void **ens)
             ens_free(e);
             return ens_err(nullptr, ICAR_ADV_EINVAL, "NULL handle %d", k);
         }
-        if (h->o.kernel_variant != 1 || h->o.rk3 || h->o.mpdata_order < 1 || h->nspecies == 0 ||
-            require_bound(h) != 0) {
+        if (h->is_std || h->o.kernel_variant != 1 || h->o.rk3 || h->o.mpdata_order < 1 ||
+            h->nspecies == 0 || require_bound(h) != 0) {
             ens_free(e);
             return ens_err(nullptr, ICAR_ADV_EINVAL,
-                           "handle %d: ensembles need kernel_variant 1, RK3 off, mpdata_order >= 1, "
-                           "bound fields and species",
+                           "handle %d: ensembles need MPDATA, kernel_variant 1, RK3 off, "
+                           "mpdata_order >= 1, bound fields and species",
                            k);
         }
         if (h->o.mpdata_order != static_cast<Handle *>(handles[0])->o.mpdata_order ||
@@ -1291,7 +1481,8 @@ const char *kernel_name(int id)
                                          "fct",         "ring_bc",    "reduce",
                                          "copy",        "fused_first_pass", "fused_corrective_pass",
                                          "convert16",   "face_coefficients", "wind_balance",
-                                         "dt_bound",    "forcing"};
+                                         "dt_bound",    "forcing",    "std_wind",
+                                         "std_stage",   "std_limiter_scale", "std_limiter_final"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "";
 }
 }  // namespace icaradv

@@ -39,6 +39,10 @@ enum KernelId {
     K_BALANCE,
     K_DT,
     K_FORCING,
+    K_STD_WIND,
+    K_STD_STAGE,
+    K_STD_SCALE,
+    K_STD_FINAL,
     K_COUNT
 };
 
@@ -89,5 +93,38 @@ void launch_copy(hipStream_t s, size_t n, const float *src, float *dst);
 // Divergence check (debug): counts cells with |div| > 1e-3; out[0]=count, out[1]=max.
 void launch_divergence(hipStream_t s, Dims d, const float *U, const float *V, const float *W,
                        const float *dz, double *partial);
+
+// ---- standard scheme, adv=1 (kernels_std.hip) ----
+// Geometry of the flux-form update: the tile (0-based memory offsets), the
+// flux orders and the per-step mass fluxes and divisors of k_std_wind.
+//   Um  x faces, (nx+1) per row (face i on the west side of cell i)
+//   Vm  y faces, ny+1 rows (face j on the south side of row j)
+//   Wm  the top face of each cell;  Rj = jacobian*rho;  Rdj = dz*jacobian*rho
+struct StdGeom {
+    Dims d;
+    int its, ite, jts, jte;
+    int horder, vorder;
+    const float *Um, *Vm, *Wm, *Rj, *Rdj;
+};
+constexpr int STD_MAX_BATCH = 16;
+// Species of one launch: the state the fluxes come from (qin), the base
+// state (qold), the output, and the limiter's work arrays.
+struct StdBatch {
+    int n;
+    const float *qin[STD_MAX_BATCH];
+    const float *qold[STD_MAX_BATCH];
+    float *out[STD_MAX_BATCH];
+    float *dq[STD_MAX_BATCH];
+    float *sin[STD_MAX_BATCH];
+    float *sout[STD_MAX_BATCH];
+};
+void launch_std_wind(hipStream_t s, Dims d, float dx, float dt, int adv_den,
+                     const DomainFields &f, float *Um, float *Vm, float *Wm, float *Rj,
+                     float *Rdj);
+// mode 0: stage; 1: stage and dumb_q; 2: dumb_q only
+void launch_std_stage(hipStream_t s, const StdGeom &g, float t_factor, int mode,
+                      const StdBatch &b);
+void launch_std_scale(hipStream_t s, const StdGeom &g, const StdBatch &b);
+void launch_std_final(hipStream_t s, const StdGeom &g, const StdBatch &b);
 
 }  // namespace icaradv

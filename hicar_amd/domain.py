@@ -16,6 +16,7 @@ import numpy as np
 # src/constants/icar_constants.f90:419-420
 kADV_STD = 1
 kADV_MPDATA = 2
+kFLUXCOR_MONO = 1  # icar_constants.f90:422
 
 # Registration order of advected species, src/objects/domain_obj.f90:68-89.
 ADV_VAR_ORDER = ("qv", "theta", "qc", "qr", "qs", "qi", "qg", "ni", "nr", "ns", "ng",
@@ -73,6 +74,20 @@ class Options:
                 self.vars_to_advect.append(n)
 
 
+def halo_size(options: "Options") -> int:
+    """Halo width ICAR gives the grid for the advection options
+    (domain_obj.f90:2979-2989 and grid_obj.f90:153-154): ceiling(order/2)
+    with order = max(h_order, v_order), plus 2 with the WRF limiter, and 4
+    for MPDATA."""
+    ao = options.adv_options
+    order = max(ao.h_order, ao.v_order)
+    if ao.flux_corr == kFLUXCOR_MONO:
+        order += 2
+    if options.physics.advection == kADV_MPDATA:
+        order = 4
+    return -(-order // 2)
+
+
 class Domain:
     """The fields of ``domain_t`` used by ``advect`` and their bounds.
 
@@ -85,7 +100,7 @@ class Domain:
               "jacobian_w", "advection_dz")
 
     def __init__(self, nx: int, ny: int, nz: int, dx: float, ims: int = 1, jms: int = 1,
-                 kms: int = 1):
+                 kms: int = 1, halo: int = 2):
         if min(nx, ny, nz) < 3:
             raise ValueError("MPDATA needs at least 3 cells in x, y and z")
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
@@ -93,9 +108,12 @@ class Domain:
         self.ims, self.jms, self.kms = ims, jms, kms
         self.ime, self.jme, self.kme = ims + nx - 1, jms + ny - 1, kms + nz - 1
         # tile bounds as update_with_halos sets them for a one-image domain
-        # with halo 2 (grid_obj.f90:238-248)
-        self.its, self.ite = self.ims + 2, self.ime - 2
-        self.jts, self.jte = self.jms + 2, self.jme - 2
+        # (grid_obj.f90:238-248); the halo is halo_size(options), 2 for MPDATA
+        if halo < 1:
+            raise ValueError(f"halo {halo} < 1")
+        self.halo = int(halo)
+        self.its, self.ite = self.ims + halo, self.ime - halo
+        self.jts, self.jte = self.jms + halo, self.jme - halo
         z = np.zeros((ny, nz, nx), np.float32)
         self.u = np.zeros((ny, nz, nx + 1), np.float32)
         self.v = np.zeros((ny + 1, nz, nx), np.float32)

@@ -18,7 +18,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from .domain import Domain, Options
+from .domain import Domain, Options, kADV_STD, halo_size
 
 HYDRO_MASS = ("qc", "qr", "qs", "qi", "qg")
 NUMBERS = ("ni", "nr", "ns", "ng")
@@ -113,11 +113,14 @@ def _face_y(a: np.ndarray) -> np.ndarray:
     return f
 
 
-def _diagnose_w(dom: Domain, u: np.ndarray, v: np.ndarray, advect_density: bool) -> np.ndarray:
+def _diagnose_w(dom: Domain, u: np.ndarray, v: np.ndarray, advect_density: bool,
+                std: bool = False) -> np.ndarray:
     """w such that the MPDATA Courant numbers of mpdata_compute_wind
     (adv_mpdata.f90:492-498) satisfy
         (U(i+1)-U(i)) + (V(j+1)-V(j)) + (W(k)-W(k-1))/dz_k = 0,  W(-1) = 0,
-    in float64 (per unit dt)."""
+    in float64 (per unit dt).  With ``std`` the same for the mass fluxes of
+    adv_std_compute_wind (advect.f90:475-484), whose interface density is
+    dz-weighted, as balance_uvw gives it (wind.f90:177)."""
     ny, nz, nx = dom.shape
     rho = dom.density.astype(np.float64) if advect_density else np.ones((ny, nz, nx))
     ru = _face_x(rho)
@@ -128,7 +131,10 @@ def _diagnose_w(dom: Domain, u: np.ndarray, v: np.ndarray, advect_density: bool)
     dz = dom.advection_dz.astype(np.float64)
     Wr = -np.cumsum(dz * divh, axis=1)
     rz = np.empty_like(rho)
-    rz[:, :nz - 1] = 0.5 * (rho[:, 1:] + rho[:, :-1])
+    if std:
+        rz[:, :nz - 1] = (rho[:, :-1] * dz[:, :-1] + rho[:, 1:] * dz[:, 1:]) / (dz[:, :-1] + dz[:, 1:])
+    else:
+        rz[:, :nz - 1] = 0.5 * (rho[:, 1:] + rho[:, :-1])
     rz[:, nz - 1] = rho[:, nz - 1]
     return Wr / (dom.jacobian_w * rz)
 
@@ -163,13 +169,14 @@ def _species(config: str, d: int, name: str, shape, z: np.ndarray,
 
 
 def _finish(name: str, dom: Domain, opts: Options, u, v, density, names, z, steps,
-            bc_x, bc_y, cfg_key, d_index=0, qv_profile=False, meta=None) -> Case:
+            bc_x, bc_y, cfg_key, d_index=0, qv_profile=False, meta=None, cfl=0.5) -> Case:
     dom.density[...] = density
-    w = _diagnose_w(dom, u, v, opts.parameters.advect_density)
+    w = _diagnose_w(dom, u, v, opts.parameters.advect_density,
+                    std=opts.physics.advection == kADV_STD)
     dom.u[...] = u
     dom.v[...] = v
     dom.w[...] = w
-    dt = courant_dt(dom, u, v, w)
+    dt = courant_dt(dom, u, v, w, cfl)
     opts.vars_to_advect = list(names)
     values = {n: _species(cfg_key, d_index, n, dom.shape, z, qv_profile=qv_profile)
               for n in names}
@@ -234,6 +241,29 @@ def config2(nx=1024, ny=1024, nz=30, steps=50, nspecies=10) -> Case:
     u, v = _fractal_winds(dom, terrain, z, 15.0)
     return _finish("C2", dom, opts, u, v, 1.2 * np.exp(-z / 8000.0), C2_SPECIES[:nspecies], z,
                    steps, 1, 2, "C2")
+
+
+def config_std(nx=1024, ny=1024, nz=30, steps=50, nspecies=10, h_order=5, v_order=3,
+               flux_corr=1, rk3=True, cfl=1.2) -> Case:
+    """The C2 domain (fractal terrain, 15 m/s westerly with terrain-forced
+    perturbation, the same species) under the standard scheme with the
+    settings of run/complete_icar_options.nml: h_order 5, v_order 3, the WRF
+    limiter and RK3 at a Courant number of 1.2 (below the 1.3 of that
+    namelist, which options_check caps at 1.4 for order 5).  The halo is
+    the one ICAR gives these options; w balances the scheme's mass fluxes."""
+    opts = Options()
+    opts.physics.advection = kADV_STD
+    opts.adv_options.h_order = h_order
+    opts.adv_options.v_order = v_order
+    opts.adv_options.flux_corr = flux_corr
+    opts.time_options.RK3 = rk3
+    opts.parameters.advect_density = True
+    dom = Domain(nx, ny, nz, 1000.0, halo=halo_size(opts))
+    terrain = fractal_terrain(nx, ny, 0.0, _cap(3000.0, nz), rng("C2", 0, "terrain"))
+    z = _metric(dom, terrain, dz_levels(nz))
+    u, v = _fractal_winds(dom, terrain, z, 15.0)
+    return _finish("STD", dom, opts, u, v, 1.2 * np.exp(-z / 8000.0), C2_SPECIES[:nspecies], z,
+                   steps, 1, 2, "C2", cfl=cfl)
 
 
 C3_SPECIES = ("qv", "qc", "qr", "qs", "qi", "qg")

@@ -1,10 +1,12 @@
 /*
- * icar_adv.h -- C-ABI of the MI355X MPDATA advection library (libicar_adv.so).
+ * icar_adv.h -- C-ABI of the MI355X advection library (libicar_adv.so).
  *
- * It replaces, for the MPDATA scheme (&physics adv=2), the three calls that
- * ICAR's advection module exposes:
+ * It replaces, for the MPDATA scheme (&physics adv=2) and the standard
+ * flux-form scheme with RK3 and the WRF limiter (adv=1), the three calls
+ * that ICAR's advection module exposes:
  *   adv_init(domain, options)      src/physics/advection_driver.f90:26-44
  *                                  -> mpdata_init, src/physics/adv_mpdata.f90:415-442
+ *                                  or adv_std_init, src/physics/advect.f90:29-71
  *   advect(domain, options, dt)    src/physics/advection_driver.f90:59-126
  *                                  (called at src/main/time_step.f90:373)
  *   adv_var_request(options)       src/physics/advection_driver.f90:46-57
@@ -43,7 +45,7 @@ typedef struct {
     float dx;
 } icar_adv_grid;
 
-/* Namelist switches the MPDATA path reads (options_t):
+/* Namelist switches the advection path reads (options_t):
  *   mpdata_order    &adv_parameters mpdata_order   (options_obj.f90:1568,1581; default 2)
  *   mpdata_fct      &adv_parameters MPDATA_FCT     (options_obj.f90:1568,1580; default .true.)
  *   advect_density  &parameters advect_density     (options_obj.f90:1025; default .false.)
@@ -55,7 +57,20 @@ typedef struct {
  *                   and species, 1 = fused kernels batched over all species
  *                   of the handle (about twice as fast); both keep the reference's
  *                   IEEE FP32 operation order and give bit-identical results
- *                   for FP32 storage. */
+ *                   for FP32 storage.  MPDATA only.
+ *   advection       &physics adv (options_obj.f90:470-490): 1 the standard
+ *                   flux-form scheme (adv_std, advect.f90), 2 MPDATA; 0 = 2
+ *   h_order,        &adv_parameters h_order / v_order (options_obj.f90:1568,
+ *   v_order         1583-1584; default 1): 1, 3 or 5 (flux3, advect.f90:
+ *                   134-257).  The grid needs (h_order+1)/2 halo cells around
+ *                   the tile, one more with the limiter; v_order 5 needs 4
+ *                   levels.
+ *   flux_corr       &adv_parameters flux_corr (:1582; default 0): 1 applies
+ *                   the WRF monotonic limiter (adv_fluxcorr.f90) in the last
+ *                   call of each step
+ * With advection 1, rk3 selects the three-stage RK3 of advection_driver.f90:
+ * 98-110 (t_factor 0.333, 0.5, 1) instead of one Euler call; the species
+ * must be stored in FP32. */
 typedef struct {
     int mpdata_order;
     int mpdata_fct;
@@ -63,7 +78,15 @@ typedef struct {
     int debug;
     int rk3;
     int kernel_variant;
+    int advection;
+    int h_order;
+    int v_order;
+    int flux_corr;
 } icar_adv_opts;
+
+/* &physics adv= (icar_constants.f90:419-420); 0 in icar_adv_opts.advection
+ * means MPDATA. */
+enum { ICAR_ADV_STD = 1, ICAR_ADV_MPDATA = 2 };
 
 /* Fields bound with icar_adv_bind_field (names of domain_t components). */
 enum {

@@ -1,4 +1,5 @@
-"""ctypes wrapper of the C oracle (oracle/mpdata_oracle.c).
+"""ctypes wrapper of the C oracle (oracle/mpdata_oracle.c, std_oracle.c,
+dyn_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the
 CPU-baseline leg of bench.py, never by the product package.  Arrays follow
@@ -128,6 +129,88 @@ class Oracle:
             for q in cur.values():
                 self.apply_bc(q, bc_x, bc_y)
         return cur
+
+    # -- standard scheme, adv=1 (std_oracle.c) ----------------------------
+    @staticmethod
+    def tile(dom):
+        """The tile bounds its..ite, jts..jte as 0-based memory offsets."""
+        return (dom.its - dom.ims, dom.ite - dom.ims, dom.jts - dom.jms, dom.jte - dom.jms)
+
+    def std_compute_wind(self, dom, dt, advect_density):
+        """adv_std_compute_wind (advect.f90:444-492): U_m on x faces
+        (ny, nz, nx+1), V_m on y faces (ny+1, nz, nx), W_m and rho on cells."""
+        Um = np.zeros(dom.field_shape("u"), self.dtype)
+        Vm = np.zeros(dom.field_shape("v"), self.dtype)
+        Wm = np.zeros(dom.shape, self.dtype)
+        rho = np.zeros(dom.shape, self.dtype)
+        a = [self.arr(getattr(dom, n)) for n in
+             ("u", "v", "w", "density", "advection_dz", "jacobian_u", "jacobian_v",
+              "jacobian_w")]
+        self.lib.oracle_std_compute_wind(dom.nx, dom.ny, dom.nz, self.real(dom.dx),
+                                         self.real(dt), int(advect_density),
+                                         *[self._p(x) for x in a], self._p(Um), self._p(Vm),
+                                         self._p(Wm), self._p(rho))
+        return Um, Vm, Wm, rho
+
+    def std_advect3d(self, dom, qf, qold, winds, horder, vorder, t_factor=1.0, flux_corr=0,
+                     fc_bounds=None):
+        """adv_std_advect3d (advect.f90:259-320): returns the new qfluxes.
+        winds = std_compute_wind(...); fc_bounds defaults to flux_corr == 1."""
+        Um, Vm, Wm, rho = winds
+        qf = self.arr(qf).copy()
+        qold = self.arr(qold)
+        fcb = (flux_corr == 1) if fc_bounds is None else bool(fc_bounds)
+        dz, jaco = self.arr(dom.advection_dz), self.arr(dom.jacobian)
+        self.lib.oracle_std_advect3d(dom.nx, dom.ny, dom.nz, *self.tile(dom), int(horder),
+                                     int(vorder), int(fcb), self._p(qf), self._p(qold),
+                                     self._p(dz), self._p(jaco), self._p(Um), self._p(Vm),
+                                     self._p(Wm), self._p(rho), self.real(t_factor),
+                                     int(flux_corr))
+        return qf
+
+    def std_advect(self, dom, options, dt, species=None):
+        """advection_driver.f90:59-126 with adv=1: the winds once, then each
+        species by RK3 or one Euler step.  Returns the dict of new species."""
+        ao = options.adv_options
+        winds = self.std_compute_wind(dom, dt, options.parameters.advect_density)
+        Um, Vm, Wm, rho = winds
+        dz, jaco = self.arr(dom.advection_dz), self.arr(dom.jacobian)
+        out = {}
+        for n, a in (dom.species if species is None else species).items():
+            q = self.arr(a).copy()
+            self.lib.oracle_std_advect_species(
+                dom.nx, dom.ny, dom.nz, *self.tile(dom), int(ao.h_order), int(ao.v_order),
+                int(ao.flux_corr), int(options.time_options.RK3), self._p(q), self._p(dz),
+                self._p(jaco), self._p(Um), self._p(Vm), self._p(Wm), self._p(rho))
+            out[n] = q
+        return out
+
+    def std_flux3(self, q, u, v, w, horder, vorder, i_s, i_e, j_s, j_e):
+        """flux3 (advect.f90:134-257) over cells i_s..i_e x j_s..j_e (0-based,
+        inclusive); u on x faces, v on y faces, w on cells.  Returns fx
+        (ny, nz, nx+1), fy (ny+1, nz, nx), fz (ny, nz+1, nx); faces outside
+        the range are 0."""
+        nx, ny, nz = self._dims(q)
+        fx = np.zeros((ny, nz, nx + 1), self.dtype)
+        fy = np.zeros((ny + 1, nz, nx), self.dtype)
+        fz = np.zeros((ny, nz + 1, nx), self.dtype)
+        q, u, v, w = (self.arr(a) for a in (q, u, v, w))
+        self.lib.oracle_std_flux3(nx, ny, nz, i_s, i_e, j_s, j_e, int(horder), int(vorder),
+                                  self._p(q), self._p(u), self._p(v), self._p(w), self._p(fx),
+                                  self._p(fy), self._p(fz))
+        return fx, fy, fz
+
+    def std_flux_corr(self, q, u, v, w, fx, fy, fz, jaco, dz, rho, its, ite, jts, jte):
+        """WRF_flux_corr (adv_fluxcorr.f90:37-140) of the given fluxes; returns
+        the corrected copies."""
+        nx, ny, nz = self._dims(q)
+        fx, fy, fz = (self.arr(a).copy() for a in (fx, fy, fz))
+        a = [self.arr(x) for x in (q, u, v, w)]
+        b = [self.arr(x) for x in (jaco, dz, rho)]
+        self.lib.oracle_std_flux_corr(nx, ny, nz, its, ite, jts, jte, *[self._p(x) for x in a],
+                                      self._p(fx), self._p(fy), self._p(fz),
+                                      *[self._p(x) for x in b])
+        return fx, fy, fz
 
     # -- wind balance and time step (dyn_oracle.c) ------------------------
     def _winds(self, dom):

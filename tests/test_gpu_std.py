@@ -1,0 +1,177 @@
+"""The standard scheme (adv=1: flux3 orders 1/3/5, RK3, the WRF monotonic
+limiter) on the device, through the C-ABI, against the FP32 oracle bit for
+bit."""
+import numpy as np
+import pytest
+
+from hicar_amd import _lib, synthetic
+from hicar_amd.advection import Advection, adv_init, advect
+from hicar_amd.domain import Domain, Options, kADV_STD, halo_size
+
+pytestmark = pytest.mark.gpu
+
+COMBOS = [  # (h_order, v_order, flux_corr, rk3)
+    (5, 3, 1, True),   # run/complete_icar_options.nml
+    (5, 5, 1, True),
+    (3, 3, 1, True),
+    (1, 1, 1, True),
+    (5, 3, 0, True),
+    (3, 1, 0, True),
+    (1, 1, 0, False),
+    (3, 3, 0, False),
+    (5, 3, 1, False),
+    (1, 3, 1, False),
+]
+
+
+def std_case(h, v, fc, rk3, nx=41, ny=33, nz=9, cfl=1.0, names=None):
+    case = synthetic.config_std(nx=nx, ny=ny, nz=nz, steps=2, h_order=h, v_order=v, flux_corr=fc,
+                                rk3=rk3, cfl=cfl if rk3 else 0.4)
+    if names is not None:
+        d, o = case.domain, case.options
+        r = np.random.default_rng(len(names))
+        o.vars_to_advect = list(names)
+        d.set_var_lists(o, {n: r.lognormal(-9, 1.5, d.shape) * (r.random(d.shape) > 0.4)
+                            for n in names})
+    return case
+
+
+def oracle_steps(oracle32, case, steps):
+    d = case.domain
+    cur = {n: a.copy() for n, a in d.species.items()}
+    for _ in range(steps):
+        cur = oracle32.std_advect(d, case.options, case.dt, cur)
+    return cur
+
+
+@pytest.mark.parametrize("h,v,fc,rk3", COMBOS)
+def test_std_matches_oracle(oracle32, h, v, fc, rk3):
+    case = std_case(h, v, fc, rk3)
+    d = case.domain
+    init = {n: a.copy() for n, a in d.species.items()}
+    ref = oracle_steps(oracle32, case, 2)
+    st = Advection(d, case.options)
+    try:
+        st.advect(case.dt)
+        st.advect(case.dt)
+        st.download()
+    finally:
+        st.close()
+    for n in ref:
+        np.testing.assert_array_equal(d.species[n], ref[n], err_msg=n)
+        assert not np.array_equal(ref[n], init[n]), n
+
+
+def test_std_host_mirror_keeps_positivity_below_top():
+    """adv_init + advect (the host mirror) with the namelist settings.  The
+    limiter leaves the model-top flux alone, so the top level can go
+    negative; the next step's bounds of the level below then include that
+    value, so after s steps the levels below nz - s are the ones that stay
+    non-negative, up to FP32 rounding (a few 1e-9 of the field's maximum
+    at most, where a zero cell takes the difference of nearly equal fluxes;
+    the limiter-free negatives are about 1e-2 of it)."""
+    case = std_case(5, 3, 1, True, nx=60, ny=50, nz=10, cfl=1.2)
+    d = case.domain
+    init = {n: a.copy() for n, a in d.species.items()}
+    st = adv_init(d, case.options)
+    try:
+        for _ in range(3):
+            advect(d, case.options, case.dt)
+    finally:
+        st.close()
+    below = slice(0, d.nz - 3)
+    for n, a in d.species.items():
+        assert not np.array_equal(a, init[n]), n
+        if init[n].min() >= 0:
+            assert a[:, below].min() >= -1e-7 * a.max(), n
+
+
+def test_std_batches_beyond_sixteen_species(oracle32):
+    names = [f"t{k:02d}" for k in range(19)]  # two launches per pass: 16 + 3
+    case = std_case(5, 3, 1, True, nx=30, ny=27, nz=7, names=names)
+    d = case.domain
+    ref = oracle_steps(oracle32, case, 1)
+    st = Advection(d, case.options)
+    try:
+        st.advect(case.dt)
+        st.download()
+    finally:
+        st.close()
+    for n in ref:
+        np.testing.assert_array_equal(d.species[n], ref[n], err_msg=n)
+
+
+@pytest.mark.parametrize("fc", [1, 0])
+def test_std_graph_replay_follows_buffer_parity(oracle32, fc):
+    """Each step swaps every species with its next-state buffer; a recorded
+    run replays only from the parity it was recorded at.  3 + 2 + 2 steps
+    (the second 2-step call replays) against 7 oracle steps."""
+    case = std_case(5, 3, fc, True, nx=36, ny=30, nz=8)
+    d = case.domain
+    ref = oracle_steps(oracle32, case, 7)
+    st = Advection(d, case.options)
+    try:
+        st.set_graph(True)
+        st.run(case.dt, 3)
+        st.run(case.dt, 2)
+        st.run(case.dt, 2)
+        st.download()
+        again = st.mass_budget()
+        assert np.isfinite(again).all()
+    finally:
+        st.close()
+    for n in ref:
+        np.testing.assert_array_equal(d.species[n], ref[n], err_msg=n)
+
+
+def test_std_profile_names_kernels():
+    case = std_case(5, 3, 1, True, nx=30, ny=26, nz=6)
+    st = Advection(case.domain, case.options)
+    try:
+        st.profile(True)
+        st.advect(case.dt)
+        st.sync()
+        ks = st.kernel_stats()
+    finally:
+        st.close()
+    assert ks["std_wind"]["launches"] == 1
+    assert ks["std_stage"]["launches"] == 2
+    assert ks["std_limiter_scale"]["launches"] == 1
+    assert ks["std_limiter_final"]["launches"] == 1
+
+
+def rejected(opts_fn, nx=30, ny=26, nz=6, halo=None):
+    o = Options()
+    o.physics.advection = kADV_STD
+    opts_fn(o)
+    d = Domain(nx, ny, nz, 1000.0, halo=halo or halo_size(o))
+    o.vars_to_advect = ["qv"]
+    d.set_var_lists(o, {"qv": np.ones(d.shape)})
+    with pytest.raises(_lib.IcarAdvError) as e:
+        Advection(d, o).close()
+    return e.value
+
+
+def test_std_rejects_bad_options():
+    def order5_fc(o):
+        o.adv_options.h_order = 5
+        o.adv_options.flux_corr = 1
+    assert rejected(order5_fc, halo=3).code == _lib.ESHAPE  # needs 4
+    def order4(o):
+        o.adv_options.h_order = 4
+    assert rejected(order4).code == _lib.EINVAL
+
+    def v5(o):
+        o.adv_options.v_order = 5
+    assert rejected(v5, nz=3).code == _lib.ESHAPE
+
+    def fc2(o):
+        o.adv_options.flux_corr = 2
+    assert rejected(fc2).code == _lib.EINVAL
+
+
+def test_std_rejects_16bit_storage():
+    case = std_case(3, 3, 0, True, nx=24, ny=20, nz=5)
+    case.domain.storage["qv"] = (_lib.STORE_BF16, 0)
+    with pytest.raises(_lib.IcarAdvError):
+        Advection(case.domain, case.options).close()
