@@ -77,12 +77,12 @@ struct Handle {
     // standard scheme (adv=1): mass fluxes, divisors, and per species the
     // buffer the next state is written to (swapped with the species' own
     // after each step: std_par counts the swaps mod 2), the second RK3 stage
-    // and the limiter's dumb_q / scale_in / scale_out
+    // and the limiter's dumb_q and (scale_in, scale_out) pairs
     bool is_std = false;
     float *Um = nullptr, *Vm = nullptr, *Wm = nullptr, *Rj = nullptr, *Rdj = nullptr;
     float *std_alt[ICAR_ADV_MAX_SPECIES] = {}, *std_b[ICAR_ADV_MAX_SPECIES] = {};
-    float *std_dq[ICAR_ADV_MAX_SPECIES] = {}, *std_si[ICAR_ADV_MAX_SPECIES] = {};
-    float *std_so[ICAR_ADV_MAX_SPECIES] = {};
+    float *std_dq[ICAR_ADV_MAX_SPECIES] = {};
+    float2 *std_sc[ICAR_ADV_MAX_SPECIES] = {};
     int std_par = 0;
     // profiling
     bool prof = false;
@@ -220,8 +220,8 @@ void free_all(Handle *h)
     float *sbufs[] = {h->Um, h->Vm, h->Wm, h->Rj, h->Rdj};
     for (float *p : sbufs) (void)hipFree(p);
     for (int s = 0; s < ICAR_ADV_MAX_SPECIES; s++) {
-        float *per[] = {h->std_alt[s], h->std_b[s], h->std_dq[s], h->std_si[s], h->std_so[s]};
-        for (float *p : per) (void)hipFree(p);
+        void *per[] = {h->std_alt[s], h->std_b[s], h->std_dq[s], h->std_sc[s]};
+        for (void *p : per) (void)hipFree(p);
     }
     (void)hipFree(h->d_filter);
     (void)hipFree(h->d_partial);
@@ -390,8 +390,7 @@ int advect_std(Handle *h, float dt)
                 b.qold[m] = home;
                 b.qin[m] = which_in == 0 ? home : which_in == 1 ? h->std_alt[s] : h->std_b[s];
                 b.dq[m] = h->std_dq[s];
-                b.sin[m] = h->std_si[s];
-                b.sout[m] = h->std_so[s];
+                b.sc[m] = h->std_sc[s];
             }
         };
         auto outs = [&](int which_out) {
@@ -401,8 +400,17 @@ int advect_std(Handle *h, float dt)
             }
         };
         // algorithmic bytes: winds and divisors once, then per species the
-        // arrays the kernel reads or writes once each
+        // arrays the kernel reads or writes once each (the scale pair counts 2)
         const double wb = 5.0 * 4 * n;
+        // the cells outside the tile keep the base state in every call's
+        // output (qfluxes = qold, advect.f90:298), dumb_q outside its ring
+        fill(0);
+        outs(1);
+        timed(h, K_STD_RING, 0.0, [&] { launch_std_ring(h->stream, g, b, fc ? 1 : 0); });
+        if (rk3) {
+            outs(2);
+            timed(h, K_STD_RING, 0.0, [&] { launch_std_ring(h->stream, g, b, 0); });
+        }
         if (rk3) {  // advection_driver.f90:103-110
             fill(0);
             outs(1);
@@ -566,6 +574,12 @@ int std_check(const icar_adv_grid *g, const icar_adv_opts *o)
                        o->h_order, o->flux_corr == 1 ? " with the limiter" : "", need, hw, hs);
     if (o->v_order == 5 && g->kme - g->kms + 1 < 4)
         return set_err(nullptr, ICAR_ADV_ESHAPE, "v_order 5 needs at least 4 levels");
+    // the kernels form byte offsets of the (scale_in, scale_out) pairs in 32 bits
+    const long long cells = (long long)(g->ime - g->ims + 2) * (g->jme - g->jms + 2) *
+                            (g->kme - g->kms + 1);
+    if (cells >= (1LL << 29))
+        return set_err(nullptr, ICAR_ADV_ERANGE,
+                       "the standard scheme takes at most 2^29 cells per domain (%lld)", cells);
     return 0;
 }
 
@@ -720,13 +734,13 @@ int icar_adv_bind_species(void *handle, int idx, float *host, int storage, int p
         if (rc) return rc;
     }
     if (h->is_std) {  // next-state buffer, second RK3 stage, limiter work arrays
-        float **per[] = {&h->std_alt[idx], &h->std_b[idx], &h->std_dq[idx], &h->std_si[idx],
-                         &h->std_so[idx]};
-        const bool need[] = {true, h->o.rk3 != 0, h->o.flux_corr == 1, h->o.flux_corr == 1,
-                             h->o.flux_corr == 1};
-        for (int q = 0; q < 5; q++)
+        void **per[] = {(void **)&h->std_alt[idx], (void **)&h->std_b[idx],
+                        (void **)&h->std_dq[idx], (void **)&h->std_sc[idx]};
+        const bool need[] = {true, h->o.rk3 != 0, h->o.flux_corr == 1, h->o.flux_corr == 1};
+        const size_t elem[] = {4, 4, 4, 8};
+        for (int q = 0; q < 4; q++)
             if (need[q] && !*per[q]) {
-                rc = dalloc(h, (void **)per[q], h->N * sizeof(float) + 64);
+                rc = dalloc(h, per[q], h->N * elem[q] + 64);
                 if (rc) return rc;
             }
     }
@@ -1482,7 +1496,8 @@ const char *kernel_name(int id)
                                          "copy",        "fused_first_pass", "fused_corrective_pass",
                                          "convert16",   "face_coefficients", "wind_balance",
                                          "dt_bound",    "forcing",    "std_wind",
-                                         "std_stage",   "std_limiter_scale", "std_limiter_final"};
+                                         "std_stage",   "std_limiter_scale", "std_limiter_final",
+                                         "std_ring"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "";
 }
 }  // namespace icaradv

@@ -43,6 +43,7 @@ enum KernelId {
     K_STD_STAGE,
     K_STD_SCALE,
     K_STD_FINAL,
+    K_STD_RING,
     K_COUNT
 };
 
@@ -108,15 +109,15 @@ struct StdGeom {
 };
 constexpr int STD_MAX_BATCH = 16;
 // Species of one launch: the state the fluxes come from (qin), the base
-// state (qold), the output, and the limiter's work arrays.
+// state (qold), the output, and the limiter's work arrays: dumb_q and the
+// (scale_in, scale_out) pair of each cell.
 struct StdBatch {
     int n;
     const float *qin[STD_MAX_BATCH];
     const float *qold[STD_MAX_BATCH];
     float *out[STD_MAX_BATCH];
     float *dq[STD_MAX_BATCH];
-    float *sin[STD_MAX_BATCH];
-    float *sout[STD_MAX_BATCH];
+    float2 *sc[STD_MAX_BATCH];
 };
 void launch_std_wind(hipStream_t s, Dims d, float dx, float dt, int adv_den,
                      const DomainFields &f, float *Um, float *Vm, float *Wm, float *Rj,
@@ -126,5 +127,8 @@ void launch_std_stage(hipStream_t s, const StdGeom &g, float t_factor, int mode,
                       const StdBatch &b);
 void launch_std_scale(hipStream_t s, const StdGeom &g, const StdBatch &b);
 void launch_std_final(hipStream_t s, const StdGeom &g, const StdBatch &b);
+// the cells no stage writes: qold into out outside the tile and, with
+// with_dq, into dq outside the tile plus one ring
+void launch_std_ring(hipStream_t s, const StdGeom &g, const StdBatch &b, int with_dq);
 
 }  // namespace icaradv
