@@ -2,8 +2,9 @@
 !! way ICAR's advection module would: domain arrays with ICAR's bounds
 !! (ims:ime, kms:kme, jms:jme; u to ime+1, v to jme+1), bound with c_loc,
 !! uploaded once, advanced, downloaded.  Input and output are raw streams:
-!!   in : 14 int32 header (nx ny nz ims kms jms nspec order fct advect_density
-!!        nsteps bc_x bc_y variant), dx dt (real32), then u v w density
+!!   in : 20 int32 header (nx ny nz ims kms jms nspec order fct advect_density
+!!        nsteps bc_x bc_y variant halo adv h_order v_order flux_corr rk3),
+!!        dx dt (real32), then u v w density
 !!        jacobian jacobian_u jacobian_v jacobian_w advection_dz and the
 !!        species, each in Fortran order
 !!   out: the species after nsteps x (advect, ring conditions)
@@ -11,7 +12,7 @@ program test_icar_adv
     use iso_c_binding
     use icar_adv_c
     implicit none
-    integer(c_int) :: hdr(14)
+    integer(c_int) :: hdr(20)
     real(c_float) :: dx, dt
     integer :: nx, ny, nz, ims, ime, kms, kme, jms, jme, nspec, s, rc, u_in, u_out
     real(c_float), allocatable, target :: u(:,:,:), v(:,:,:), w(:,:,:), density(:,:,:), jaco(:,:,:)
@@ -37,11 +38,17 @@ program test_icar_adv
     read(u_in) u, v, w, density, jaco, jaco_u, jaco_v, jaco_w, adz, species
     close(u_in)
 
-    grid = icar_adv_grid(ims, ime, kms, kme, jms, jme, ims + 2, ime - 2, jms + 2, jme - 2, dx)
+    grid = icar_adv_grid(ims, ime, kms, kme, jms, jme, ims + hdr(15), ime - hdr(15), &
+                         jms + hdr(15), jme - hdr(15), dx)
     opts%mpdata_order = hdr(8)
     opts%mpdata_fct = hdr(9)
     opts%advect_density = hdr(10)
     opts%kernel_variant = hdr(14)
+    opts%advection = hdr(16)
+    opts%h_order = hdr(17)
+    opts%v_order = hdr(18)
+    opts%flux_corr = hdr(19)
+    opts%rk3 = hdr(20)
     rc = icar_adv_create(grid, opts, h)
     if (rc /= 0) error stop "icar_adv_create failed"
     call check(icar_adv_bind_field(h, ICAR_ADV_FIELD_U, c_loc(u), lbound(u), ubound(u)))
