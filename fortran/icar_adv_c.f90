@@ -19,6 +19,8 @@ module icar_adv_c
     public :: icar_adv_run, icar_adv_set_graph, icar_adv_error_message
     public :: icar_adv_balance_w, icar_adv_compute_dt, icar_adv_launch_config
     public :: icar_adv_set_relax_filter, icar_adv_set_forcing, icar_adv_apply_forcing
+    public :: icar_adv_halo_bytes, icar_adv_halo_pack, icar_adv_halo_unpack, icar_adv_halo_exchange
+    public :: ICAR_ADV_NORTH, ICAR_ADV_SOUTH, ICAR_ADV_EAST, ICAR_ADV_WEST
     public :: ICAR_ADV_FIELD_U, ICAR_ADV_FIELD_V, ICAR_ADV_FIELD_W, ICAR_ADV_FIELD_DENSITY
     public :: ICAR_ADV_FIELD_JACOBIAN, ICAR_ADV_FIELD_JACOBIAN_U, ICAR_ADV_FIELD_JACOBIAN_V
     public :: ICAR_ADV_FIELD_JACOBIAN_W, ICAR_ADV_FIELD_ADVECTION_DZ
@@ -38,6 +40,8 @@ module icar_adv_c
     integer(c_int), parameter :: ICAR_ADV_EUNBOUND = -3, ICAR_ADV_ERANGE = -4, ICAR_ADV_ENOMEM = -5
     integer(c_int), parameter :: ICAR_ADV_ECOURANT = -6
     integer(c_int), parameter :: ICAR_ADV_STD = 1, ICAR_ADV_MPDATA = 2  ! kADV_STD, kADV_MPDATA
+    integer(c_int), parameter :: ICAR_ADV_NORTH = 0, ICAR_ADV_SOUTH = 1, ICAR_ADV_EAST = 2
+    integer(c_int), parameter :: ICAR_ADV_WEST = 3
     integer(c_long_long), parameter :: ICAR_ADV_MASK_ALL_FIELDS = int(z'1FF', c_long_long)
     ! bits 16..63 (species 0..47); written as a negative two's-complement value
     integer(c_long_long), parameter :: ICAR_ADV_MASK_ALL_SPECIES = -65536_c_long_long
@@ -195,6 +199,41 @@ module icar_adv_c
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: handle
             real(c_double), value :: dt
+        end function
+
+        !> halo exchange of the advected species (domain%halo_exchange,
+        !! domain_obj.f90:573-650): a side's strip into device memory, device
+        !! memory into the halo on a side, or one handle's strip straight into
+        !! its neighbour's opposite halo on the same device
+        integer(c_int) function icar_adv_halo_bytes(handle, side, nbytes) &
+                bind(C, name="icar_adv_halo_bytes")
+            import :: c_int, c_ptr, c_long_long
+            type(c_ptr), value :: handle
+            integer(c_int), value :: side
+            integer(c_long_long), intent(out) :: nbytes
+        end function
+
+        integer(c_int) function icar_adv_halo_pack(handle, side, dev_buf, nbytes) &
+                bind(C, name="icar_adv_halo_pack")
+            import :: c_int, c_ptr, c_long_long
+            type(c_ptr), value :: handle, dev_buf
+            integer(c_int), value :: side
+            integer(c_long_long), value :: nbytes
+        end function
+
+        integer(c_int) function icar_adv_halo_unpack(handle, side, dev_buf, nbytes) &
+                bind(C, name="icar_adv_halo_unpack")
+            import :: c_int, c_ptr, c_long_long
+            type(c_ptr), value :: handle, dev_buf
+            integer(c_int), value :: side
+            integer(c_long_long), value :: nbytes
+        end function
+
+        integer(c_int) function icar_adv_halo_exchange(from, side, to) &
+                bind(C, name="icar_adv_halo_exchange")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: from, to
+            integer(c_int), value :: side
         end function
 
         integer(c_int) function icar_adv_launch_config(handle, corr_cfg, ksplit) &

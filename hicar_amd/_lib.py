@@ -25,6 +25,7 @@ MASK_ALL_FIELDS = 0x1FF
 MASK_ALL_SPECIES = 0xFFFFFFFFFFFF0000
 EINVAL, ESHAPE, EUNBOUND, ERANGE, ENOMEM, ECOURANT = -1, -2, -3, -4, -5, -6
 ADV_STD, ADV_MPDATA = 1, 2
+NORTH, SOUTH, EAST, WEST = 0, 1, 2, 3
 
 
 def mask_species(s: int) -> int:
@@ -41,7 +42,8 @@ EXPORTS = (
     "icar_adv_ensemble_last_error", "icar_adv_ensemble_profile", "icar_adv_ensemble_kernel_stats",
     "icar_adv_launch_config", "icar_adv_ensemble_launch_config", "icar_adv_balance_w",
     "icar_adv_compute_dt", "icar_adv_set_relax_filter", "icar_adv_set_forcing",
-    "icar_adv_apply_forcing",
+    "icar_adv_apply_forcing", "icar_adv_halo_bytes", "icar_adv_halo_pack",
+    "icar_adv_halo_unpack", "icar_adv_halo_exchange",
 )
 
 
@@ -112,6 +114,10 @@ def load() -> ctypes.CDLL:
         "icar_adv_set_forcing": ([vp, ip, fp, fp], ip),
         "icar_adv_apply_forcing": ([vp, ctypes.c_double], ip),
         "icar_adv_ensemble_launch_config": ([vp, i3, i3], ip),
+        "icar_adv_halo_bytes": ([vp, ip, ctypes.POINTER(ctypes.c_ulonglong)], ip),
+        "icar_adv_halo_pack": ([vp, ip, vp, ctypes.c_ulonglong], ip),
+        "icar_adv_halo_unpack": ([vp, ip, vp, ctypes.c_ulonglong], ip),
+        "icar_adv_halo_exchange": ([vp, ip, vp], ip),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -149,6 +149,27 @@ class Advection:
         """apply_forcing (domain_obj.f90:3493-3503) of every forced species."""
         self._check(self.lib.icar_adv_apply_forcing(self.h, ctypes.c_double(dt)))
 
+    # -- halo exchange -------------------------------------------------------
+    def halo_bytes(self, side: int) -> int:
+        """Bytes of the strip side `side` (_lib.NORTH/SOUTH/EAST/WEST) sends."""
+        n = ctypes.c_ulonglong()
+        self._check(self.lib.icar_adv_halo_bytes(self.h, side, ctypes.byref(n)))
+        return n.value
+
+    def halo_pack(self, side: int, dev_ptr: int, nbytes: int) -> None:
+        """The strip inside the tile next to `side` into device memory at
+        dev_ptr (what a send to that neighbour carries)."""
+        self._check(self.lib.icar_adv_halo_pack(self.h, side, ctypes.c_void_p(dev_ptr), nbytes))
+
+    def halo_unpack(self, side: int, dev_ptr: int, nbytes: int) -> None:
+        """Device memory at dev_ptr into the halo on `side`."""
+        self._check(self.lib.icar_adv_halo_unpack(self.h, side, ctypes.c_void_p(dev_ptr), nbytes))
+
+    def halo_exchange(self, side: int, to: "Advection") -> None:
+        """This domain's strip on `side` into `to`'s halo on the opposite side
+        (both on one device)."""
+        self._check(self.lib.icar_adv_halo_exchange(self.h, side, to.h))
+
     def set_graph(self, enable: bool) -> None:
         """Record the run() loop in a HIP graph and replay it."""
         self._check(self.lib.icar_adv_set_graph(self.h, int(enable)))

@@ -552,6 +552,74 @@ int apply_bc_on(Handle *h, T *owner, hipStream_t stream, int bc_x, int bc_y)
     return 0;
 }
 
+// The strip a side of the tile sends (inside the tile, as wide as the halo
+// on that side) and the halo it fills on the neighbour's opposite side,
+// both over the tile's range along the side (exchangeable_obj.f90:240-392,
+// domain_obj.f90:573-650: corners are not exchanged).  0-based offsets.
+struct Box {
+    int i0, ni, j0, nj;
+};
+
+bool side_boxes(const Handle *h, int side, Box *send, Box *recv)
+{
+    const icar_adv_grid &g = h->g;
+    const int its = g.its - g.ims, ite = g.ite - g.ims, jts = g.jts - g.jms, jte = g.jte - g.jms;
+    const int nx = h->d.nx, ny = h->d.ny, ti = ite - its + 1, tj = jte - jts + 1;
+    int w;
+    switch (side) {
+    case ICAR_ADV_NORTH:  // rows jte-w+1..jte -> the neighbour's rows jms..jts-1
+        w = ny - 1 - jte;
+        *send = {its, ti, jte - w + 1, w};
+        *recv = {its, ti, jte + 1, w};
+        break;
+    case ICAR_ADV_SOUTH:
+        w = jts;
+        *send = {its, ti, jts, w};
+        *recv = {its, ti, 0, w};
+        break;
+    case ICAR_ADV_EAST:
+        w = nx - 1 - ite;
+        *send = {ite - w + 1, w, jts, tj};
+        *recv = {ite + 1, w, jts, tj};
+        break;
+    case ICAR_ADV_WEST:
+        w = its;
+        *send = {its, w, jts, tj};
+        *recv = {0, w, jts, tj};
+        break;
+    default:
+        return false;
+    }
+    return w >= 1 && w <= ti && w <= tj;
+}
+
+int opposite(int side)
+{
+    return side == ICAR_ADV_NORTH ? ICAR_ADV_SOUTH
+           : side == ICAR_ADV_SOUTH ? ICAR_ADV_NORTH
+           : side == ICAR_ADV_EAST  ? ICAR_ADV_WEST
+                                    : ICAR_ADV_EAST;
+}
+
+// bytes of box b for every bound species of h; fills the species table
+size_t strip_table(Handle *h, const Box &b, HaloStrips *hs, bool as_source)
+{
+    size_t bytes = 0;
+    hs->n = 0;
+    for (int s = 0; s < h->nspecies; s++) {
+        if (!h->sp[s].bound) continue;
+        const int m = hs->n++;
+        const int es = h->sp[s].storage == ICAR_ADV_STORE_F32 ? 4 : 2;
+        if (as_source) hs->a[m] = species_dev(h, s);
+        else hs->b[m] = species_dev(h, s);
+        hs->esize[m] = es;
+        hs->buf_off[m] = (long long)bytes;
+        bytes += (size_t)es * b.ni * b.nj * h->d.nz;
+        bytes = (bytes + 15) & ~(size_t)15;
+    }
+    return bytes;
+}
+
 // Options and bounds the standard scheme needs: orders 1/3/5, flux_corr 0/1,
 // and a halo wide enough for flux3's stencil (plus one cell for the limiter's
 // extended tile), as ICAR's grid gives it (domain_obj.f90:2979-2989).
@@ -1111,6 +1179,119 @@ int icar_adv_apply_forcing(void *handle, double dt)
     return 0;
 }
 
+int icar_adv_halo_bytes(void *handle, int side, unsigned long long *bytes)
+{
+    Handle *h;
+    int rc = check_handle(handle, &h);
+    if (rc) return rc;
+    if (!bytes) return set_err(h, ICAR_ADV_EINVAL, "NULL output");
+    Box snd, rcv;
+    if (!side_boxes(h, side, &snd, &rcv))
+        return set_err(h, ICAR_ADV_EINVAL, "side %d: unknown, or no halo on that side", side);
+    HaloStrips hs{};
+    *bytes = strip_table(h, snd, &hs, true);
+    return 0;
+}
+
+int icar_adv_halo_pack(void *handle, int side, void *dev_buf, unsigned long long bytes)
+{
+    Handle *h;
+    int rc = check_handle(handle, &h);
+    if (rc) return rc;
+    Box snd, rcv;
+    if (!side_boxes(h, side, &snd, &rcv))
+        return set_err(h, ICAR_ADV_EINVAL, "side %d: unknown, or no halo on that side", side);
+    HaloStrips hs{};
+    const size_t need = strip_table(h, snd, &hs, true);
+    if (!dev_buf || bytes < need)
+        return set_err(h, ICAR_ADV_ERANGE, "halo buffer of %llu bytes, side %d needs %zu", bytes,
+                       side, need);
+    hs.ni = snd.ni;
+    hs.nj = snd.nj;
+    hs.a_i0 = snd.i0;
+    hs.a_j0 = snd.j0;
+    hs.a_nx = h->d.nx;
+    hs.buf = dev_buf;
+    timed(h, K_HALO, 2.0 * need, [&] { launch_strip(h->stream, hs, h->d.nz, 0); });
+    HIPCHK(h, hipGetLastError());
+    return 0;
+}
+
+int icar_adv_halo_unpack(void *handle, int side, const void *dev_buf, unsigned long long bytes)
+{
+    Handle *h;
+    int rc = check_handle(handle, &h);
+    if (rc) return rc;
+    Box snd, rcv;
+    if (!side_boxes(h, side, &snd, &rcv))
+        return set_err(h, ICAR_ADV_EINVAL, "side %d: unknown, or no halo on that side", side);
+    HaloStrips hs{};
+    const size_t need = strip_table(h, rcv, &hs, false);
+    if (!dev_buf || bytes < need)
+        return set_err(h, ICAR_ADV_ERANGE, "halo buffer of %llu bytes, side %d needs %zu", bytes,
+                       side, need);
+    hs.ni = rcv.ni;
+    hs.nj = rcv.nj;
+    hs.b_i0 = rcv.i0;
+    hs.b_j0 = rcv.j0;
+    hs.b_nx = h->d.nx;
+    hs.buf = const_cast<void *>(dev_buf);
+    timed(h, K_HALO, 2.0 * need, [&] { launch_strip(h->stream, hs, h->d.nz, 1); });
+    HIPCHK(h, hipGetLastError());
+    return 0;
+}
+
+int icar_adv_halo_exchange(void *from, int side, void *to)
+{
+    Handle *a, *b;
+    int rc = check_handle(from, &a);
+    if (rc) return rc;
+    rc = check_handle(to, &b);
+    if (rc) return rc;
+    Box snd, rcv_a, snd_b, rcv;
+    if (!side_boxes(a, side, &snd, &rcv_a) || !side_boxes(b, opposite(side), &snd_b, &rcv))
+        return set_err(a, ICAR_ADV_EINVAL, "side %d: unknown, or no halo on one of the two sides",
+                       side);
+    HaloStrips hs{};
+    strip_table(a, snd, &hs, true);
+    HaloStrips hb{};
+    strip_table(b, rcv, &hb, false);
+    if (snd.ni != rcv.ni || snd.nj != rcv.nj || a->d.nz != b->d.nz || hs.n != hb.n)
+        return set_err(a, ICAR_ADV_ESHAPE,
+                       "neighbours do not match: strip %d x %d x %d (%d species) against %d x %d x "
+                       "%d (%d species)",
+                       snd.ni, a->d.nz, snd.nj, hs.n, rcv.ni, b->d.nz, rcv.nj, hb.n);
+    for (int m = 0; m < hs.n; m++) {
+        if (hs.esize[m] != hb.esize[m])
+            return set_err(a, ICAR_ADV_ESHAPE, "species %d is stored differently on the two sides", m);
+        hs.b[m] = hb.b[m];
+    }
+    hs.ni = snd.ni;
+    hs.nj = snd.nj;
+    hs.a_i0 = snd.i0;
+    hs.a_j0 = snd.j0;
+    hs.a_nx = a->d.nx;
+    hs.b_i0 = rcv.i0;
+    hs.b_j0 = rcv.j0;
+    hs.b_nx = b->d.nx;
+    // the copy runs on the receiver's stream after the sender's queued work;
+    // the sender's next work waits for it
+    hipEvent_t ea = pool_event(a), eb = pool_event(b);
+    if (!ea || !eb) return set_err(a, ICAR_ADV_ENOMEM, "event creation failed");
+    HIPCHK(a, hipEventRecord(ea, a->stream));
+    HIPCHK(b, hipStreamWaitEvent(b->stream, ea, 0));
+    timed(b, K_HALO, 2.0 * 4 * (double)snd.ni * snd.nj * a->d.nz * hs.n,
+          [&] { launch_strip(b->stream, hs, b->d.nz, 2); });
+    HIPCHK(b, hipGetLastError());
+    HIPCHK(b, hipEventRecord(eb, b->stream));
+    HIPCHK(a, hipStreamWaitEvent(a->stream, eb, 0));
+    // a wait holds the event's state at the time it was enqueued, so the
+    // events can be recorded again by later calls
+    a->event_pool.push_back(ea);
+    b->event_pool.push_back(eb);
+    return 0;
+}
+
 const char *icar_adv_last_error(void *handle)
 {
     if (!handle) return g_create_error.c_str();
@@ -1497,7 +1678,7 @@ const char *kernel_name(int id)
                                          "convert16",   "face_coefficients", "wind_balance",
                                          "dt_bound",    "forcing",    "std_wind",
                                          "std_stage",   "std_limiter_scale", "std_limiter_final",
-                                         "std_ring"};
+                                         "std_ring",    "halo_strip"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "";
 }
 }  // namespace icaradv

@@ -44,6 +44,7 @@ enum KernelId {
     K_STD_SCALE,
     K_STD_FINAL,
     K_STD_RING,
+    K_HALO,
     K_COUNT
 };
 
@@ -94,6 +95,24 @@ void launch_copy(hipStream_t s, size_t n, const float *src, float *dst);
 // Divergence check (debug): counts cells with |div| > 1e-3; out[0]=count, out[1]=max.
 void launch_divergence(hipStream_t s, Dims d, const float *U, const float *V, const float *W,
                        const float *dz, double *partial);
+
+// ---- halo strips (kernels_dyn.hip) ----
+// One side's strip of every species: the box [a_i0, a_i0+ni) x [a_j0, a_j0+nj)
+// of the arrays a (row length a_nx), the same-shaped box at (b_i0, b_j0) of
+// the arrays b (row length b_nx), and the packed buffer (species s at byte
+// offset buf_off[s], esize[s] bytes per element).
+constexpr int HALO_MAX_SPECIES = 48;
+struct HaloStrips {
+    int n, ni, nj;
+    int a_i0, a_j0, a_nx, b_i0, b_j0, b_nx;
+    const void *a[HALO_MAX_SPECIES];
+    void *b[HALO_MAX_SPECIES];
+    void *buf;
+    long long buf_off[HALO_MAX_SPECIES];
+    int esize[HALO_MAX_SPECIES];
+};
+// mode 0: a -> buffer; 1: buffer -> b; 2: a -> b
+void launch_strip(hipStream_t s, const HaloStrips &hs, int nz, int mode);
 
 // ---- standard scheme, adv=1 (kernels_std.hip) ----
 // Geometry of the flux-form update: the tile (0-based memory offsets), the

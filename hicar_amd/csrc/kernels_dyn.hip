@@ -217,3 +217,56 @@ void launch_forcing(hipStream_t s, Dims d, double dt, const float *filt, float *
 }
 
 }  // namespace icaradv
+
+namespace icaradv {
+
+namespace {
+
+// Copies one rectangular strip [i0, i0+ni) x [j0, j0+nj), all levels, of
+// every species between the species arrays and a packed buffer, or from one
+// handle's species to another's (both strips the same shape).  Elements are
+// moved as raw 4- or 2-byte words, so 16-bit storage travels as it is held.
+// Buffer order: species, then j, k, i (i fastest): each species' strip is a
+// contiguous sub-box of its (i, k, j) array.
+__global__ void __launch_bounds__(256)
+    k_strip(HaloStrips hs, int nz, long long per_species_cells, int mode)
+{
+    const long long total = per_species_cells * hs.n;
+    for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < total;
+         t += (long long)gridDim.x * 256) {
+        const int s = (int)(t / per_species_cells);
+        long long r = t - (long long)s * per_species_cells;
+        const int ii = (int)(r % hs.ni);
+        r /= hs.ni;
+        const int k = (int)(r % nz);
+        const int jj = (int)(r / nz);
+        const size_t a = (size_t)(hs.a_i0 + ii) + (size_t)hs.a_nx * ((size_t)k + (size_t)nz * (hs.a_j0 + jj));
+        const size_t b = (size_t)(hs.b_i0 + ii) + (size_t)hs.b_nx * ((size_t)k + (size_t)nz * (hs.b_j0 + jj));
+        const int es = hs.esize[s];
+        char *boff = (char *)hs.buf + (size_t)hs.buf_off[s];
+        if (mode == 0) {  // species a -> buffer
+            if (es == 4) ((float *)boff)[t - (long long)s * per_species_cells] = ((const float *)hs.a[s])[a];
+            else ((uint16_t *)boff)[t - (long long)s * per_species_cells] = ((const uint16_t *)hs.a[s])[a];
+        } else if (mode == 1) {  // buffer -> species b
+            if (es == 4) ((float *)hs.b[s])[b] = ((const float *)boff)[t - (long long)s * per_species_cells];
+            else ((uint16_t *)hs.b[s])[b] = ((const uint16_t *)boff)[t - (long long)s * per_species_cells];
+        } else {  // species a -> species b
+            if (es == 4) ((float *)hs.b[s])[b] = ((const float *)hs.a[s])[a];
+            else ((uint16_t *)hs.b[s])[b] = ((const uint16_t *)hs.a[s])[a];
+        }
+    }
+}
+
+}  // namespace
+
+void launch_strip(hipStream_t s, const HaloStrips &hs, int nz, int mode)
+{
+    const long long per = (long long)hs.ni * hs.nj * nz;
+    const long long total = per * hs.n;
+    if (total <= 0) return;
+    long long blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_strip, dim3((unsigned)blocks), dim3(256), 0, s, hs, nz, per, mode);
+}
+
+}  // namespace icaradv

@@ -118,6 +118,9 @@ enum {
     ICAR_ADV_STORE_F16_SR = 4
 };
 
+/* Sides of the tile for the halo exchange. */
+enum { ICAR_ADV_NORTH = 0, ICAR_ADV_SOUTH = 1, ICAR_ADV_EAST = 2, ICAR_ADV_WEST = 3 };
+
 /* Error codes (negative); positive values are hipError_t. */
 enum {
     ICAR_ADV_OK = 0,
@@ -212,6 +215,31 @@ int icar_adv_compute_dt(void *handle, float cfl, int cfl_strictness, const float
  * forcing += dqdt*dt everywhere; where the filter is 1 the species takes the
  * forcing, where it is below 1 it relaxes by filter*dt/3600 of the
  * difference (double precision intermediates, as dt%seconds() is). */
+/* Halo exchange of the advected species between neighbouring subdomains
+ * (domain%halo_exchange / halo_exchange_batch before advect,
+ * time_step.f90:362-366; domain_obj.f90:573-650, exchangeable_obj.f90:
+ * 240-392).  As in the reference, a side sends the strip inside the tile as
+ * wide as its halo (rows jte-h+1..jte for the north side) over the tile's
+ * range along the side, and the neighbour writes it into its halo on the
+ * opposite side (rows jms..jts-1); corners are not exchanged.
+ *   icar_adv_halo_bytes   bytes of one side's strip for every bound species
+ *                         (species after species, each its (i, k, j) sub-box
+ *                         at its storage width, 16-byte aligned)
+ *   icar_adv_halo_pack    the strip into dev_buf (device memory), on the
+ *                         handle's stream: what an RCCL send of this side
+ *                         would carry
+ *   icar_adv_halo_unpack  dev_buf into the halo on that side
+ *   icar_adv_halo_exchange  from's strip on `side` straight into to's halo
+ *                         on the opposite side, both handles on one device
+ *                         (subdomains that share a GPU); ordered after the
+ *                         sender's queued work, and the sender's later work
+ *                         after the copy
+ * Both sides need the same species, storage kinds and strip shape. */
+int icar_adv_halo_bytes(void *handle, int side, unsigned long long *bytes);
+int icar_adv_halo_pack(void *handle, int side, void *dev_buf, unsigned long long bytes);
+int icar_adv_halo_unpack(void *handle, int side, const void *dev_buf, unsigned long long bytes);
+int icar_adv_halo_exchange(void *from, int side, void *to);
+
 int icar_adv_set_relax_filter(void *handle, int west, int east, int north, int south, int halo);
 int icar_adv_set_forcing(void *handle, int idx, const float *forcing_hi, const float *dqdt);
 int icar_adv_apply_forcing(void *handle, double dt);

@@ -62,6 +62,24 @@ def test_std_matches_oracle(oracle32, h, v, fc, rk3):
         assert not np.array_equal(ref[n], init[n]), n
 
 
+@pytest.mark.parametrize("h,v,fc,rk3", [(5, 3, 1, True), (5, 5, 1, True), (3, 3, 0, True),
+                                        (1, 3, 1, False)])
+def test_std_tall_column_chunks_match_oracle(oracle32, h, v, fc, rk3):
+    """21 levels: the limiter kernels march each column in two chunks, the
+    second starting from the face under its first level."""
+    case = std_case(h, v, fc, rk3, nx=36, ny=28, nz=21)
+    d = case.domain
+    ref = oracle_steps(oracle32, case, 2)
+    st = Advection(d, case.options)
+    try:
+        st.run(case.dt, 2)
+        st.download()
+    finally:
+        st.close()
+    for n in ref:
+        np.testing.assert_array_equal(d.species[n], ref[n], err_msg=n)
+
+
 def test_std_host_mirror_keeps_positivity_below_top():
     """adv_init + advect (the host mirror) with the namelist settings.  The
     limiter leaves the model-top flux alone, so the top level can go
