@@ -64,3 +64,64 @@ def test_fullsize_fused_equals_per_pass_and_properties(workload):
     again, m2 = run(case, init, 1)
     np.testing.assert_array_equal(m1, m2)
     np.testing.assert_array_equal(m0, m1)
+
+
+STD_STEPS = 2
+
+
+@pytest.fixture(scope="module")
+def std_workload():
+    case = synthetic.config_std(steps=STD_STEPS)
+    d = case.domain
+    case.options.vars_to_advect.append("const")
+    values = dict(d.species)
+    values["const"] = np.full(d.shape, 2.5, np.float32)
+    d.set_var_lists(case.options, values)
+    init = {n: a.copy() for n, a in d.species.items()}
+    return case, init
+
+
+def run_std(case, init, graph=False):
+    d = case.domain
+    for n, a in init.items():
+        d.species[n][...] = a
+    st = MpdataAdvection(d, case.options)
+    try:
+        st.set_graph(graph)
+        st.run(case.dt, STD_STEPS)
+        st.download()
+    finally:
+        st.close()
+    return {n: d.species[n].copy() for n in d.species}
+
+
+def test_fullsize_standard_scheme_properties(std_workload):
+    """The 1024 x 1024 x 30 domain with 10 species plus a constant tracer
+    under the standard scheme (h_order 5, v_order 3, WRF limiter, RK3,
+    Courant 1.2), two steps:
+      - the synthetic w balances the scheme's mass fluxes, so the constant
+        tracer stays constant to rounding;
+      - below the levels the unlimited model-top flux can reach in two
+        steps (2s - 1 = 3: the two upwind half steps carry it two levels
+        down per step), species that start non-negative stay so (to FP32
+        rounding of a zero cell, 1e-7 of the field's maximum);
+      - the cells outside the tile keep their values (qfluxes = qold);
+      - eager and graph-replayed runs give the same bits."""
+    case, init = std_workload
+    d = case.domain
+    out = run_std(case, init)
+    c = out["const"].astype(np.float64)
+    assert np.abs(c / 2.5 - 1).max() < 1e-5, np.abs(c / 2.5 - 1).max()
+    below = slice(0, d.nz - (2 * STD_STEPS - 1))
+    for n, a in out.items():
+        if init[n].min() >= 0:
+            assert a[:, below].min() >= -1e-7 * a.max(), (n, float(a[:, below].min()))
+    i0, i1, j0, j1 = d.its - d.ims, d.ite - d.ims, d.jts - d.jms, d.jte - d.jms
+    ring = np.ones(d.shape, bool)
+    ring[j0:j1 + 1, :, i0:i1 + 1] = False
+    for n, a in out.items():
+        np.testing.assert_array_equal(a[ring], init[n][ring], err_msg=n)
+        assert not np.array_equal(a, init[n]) or n == "const", n
+    again = run_std(case, init, graph=True)
+    for n in out:
+        np.testing.assert_array_equal(again[n], out[n], err_msg=n)

@@ -83,11 +83,12 @@ def test_std_tall_column_chunks_match_oracle(oracle32, h, v, fc, rk3):
 def test_std_host_mirror_keeps_positivity_below_top():
     """adv_init + advect (the host mirror) with the namelist settings.  The
     limiter leaves the model-top flux alone, so the top level can go
-    negative; the next step's bounds of the level below then include that
-    value, so after s steps the levels below nz - s are the ones that stay
-    non-negative, up to FP32 rounding (a few 1e-9 of the field's maximum
-    at most, where a zero cell takes the difference of nearly equal fluxes;
-    the limiter-free negatives are about 1e-2 of it)."""
+    negative; in the next step the two upwind half steps carry that value
+    two levels down (the second half step reads the first one's result), so
+    after s steps the levels below nz - (2s - 1) are the ones that stay
+    non-negative, up to FP32 rounding (a few 1e-9 of the field's maximum at
+    most, where a zero cell takes the difference of nearly equal fluxes; the
+    limiter-free negatives are about 1e-2 of it)."""
     case = std_case(5, 3, 1, True, nx=60, ny=50, nz=10, cfl=1.2)
     d = case.domain
     init = {n: a.copy() for n, a in d.species.items()}
@@ -97,7 +98,7 @@ def test_std_host_mirror_keeps_positivity_below_top():
             advect(d, case.options, case.dt)
     finally:
         st.close()
-    below = slice(0, d.nz - 3)
+    below = slice(0, d.nz - 5)
     for n, a in d.species.items():
         assert not np.array_equal(a, init[n]), n
         if init[n].min() >= 0:

@@ -21,7 +21,8 @@ invariant or a special case:
                    collapses to the two-half-step upwind flux u*q; below the
                    model top a non-negative field stays non-negative and
                    within its initial range (the top face is not limited,
-                   adv_fluxcorr.f90:139-143)
+                   adv_fluxcorr.f90:139-143; its values move two levels down
+                   a step through upwind_flux3's two half steps)
 A plausible slip (a swapped stencil offset, a wrong coefficient or sign, a
 dropped half step, t_factor applied to the base state) fails one of them.
 """
@@ -312,3 +313,21 @@ def test_top_face_is_not_limited(oracle64):
     d, o = case.domain, case.options
     a = oracle64.std_advect(d, o, case.dt, {"qc": d.species["qc"]})["qc"]
     assert a[:, d.nz - 1].min() < 0.0
+
+
+def test_top_quirk_reaches_two_levels_per_step(oracle64):
+    """Over later steps the top's out-of-bounds values enter the levels below
+    through the limiter's own upwind value: the second of upwind_flux3's two
+    half steps reads the first one's dumb_q, so a value moves two levels a
+    step.  After s steps the levels below nz - (2s - 1) stay non-negative,
+    and in this case the second step already reaches level nz - 3."""
+    case = synthetic.config_std(nx=40, ny=34, nz=10, steps=3, cfl=1.2)
+    d, o = case.domain, case.options
+    q = {n: d.species[n].astype(np.float64) for n in ("qc", "qr", "qv")}
+    reached = []
+    for s in range(1, 4):
+        q = oracle64.std_advect(d, o, case.dt, q)
+        for n, a in q.items():
+            assert a[:, :d.nz - (2 * s - 1)].min() >= 0.0, (s, n)
+        reached.append(min(k for a in q.values() for k in np.where(a.min(axis=(0, 2)) < 0)[0]))
+    assert reached[0] == d.nz - 1 and reached[1] == d.nz - 3
