@@ -43,7 +43,8 @@ EXPORTS = (
     "icar_adv_launch_config", "icar_adv_ensemble_launch_config", "icar_adv_balance_w",
     "icar_adv_compute_dt", "icar_adv_set_relax_filter", "icar_adv_set_forcing",
     "icar_adv_apply_forcing", "icar_adv_halo_bytes", "icar_adv_halo_pack",
-    "icar_adv_halo_unpack", "icar_adv_halo_exchange",
+    "icar_adv_halo_unpack", "icar_adv_halo_exchange", "icar_adv_enforce_limits",
+    "icar_adv_snapshot", "icar_adv_snapshot_wait",
 )
 
 
@@ -118,6 +119,9 @@ def load() -> ctypes.CDLL:
         "icar_adv_halo_pack": ([vp, ip, vp, ctypes.c_ulonglong], ip),
         "icar_adv_halo_unpack": ([vp, ip, vp, ctypes.c_ulonglong], ip),
         "icar_adv_halo_exchange": ([vp, ip, vp], ip),
+        "icar_adv_enforce_limits": ([vp], ip),
+        "icar_adv_snapshot": ([vp, ctypes.c_ulonglong], ip),
+        "icar_adv_snapshot_wait": ([vp], ip),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -149,6 +149,22 @@ class Advection:
         """apply_forcing (domain_obj.f90:3493-3503) of every forced species."""
         self._check(self.lib.icar_adv_apply_forcing(self.h, ctypes.c_double(dt)))
 
+    # -- the step's tail and output ------------------------------------------
+    def enforce_limits(self) -> None:
+        """domain%enforce_limits (domain_obj.f90:3092-3150) on the device
+        species: values below zero become zero."""
+        self._check(self.lib.icar_adv_enforce_limits(self.h))
+
+    def snapshot(self, species: bool = True, fields: bool = False) -> None:
+        """Starts copying the current species (and fields) out for output or
+        a restart file; the copy overlaps the following steps.
+        ``snapshot_wait`` puts the values into the bound host arrays."""
+        mask = (_lib.MASK_ALL_FIELDS if fields else 0) | (_lib.MASK_ALL_SPECIES if species else 0)
+        self._check(self.lib.icar_adv_snapshot(self.h, mask))
+
+    def snapshot_wait(self) -> None:
+        self._check(self.lib.icar_adv_snapshot_wait(self.h))
+
     # -- halo exchange -------------------------------------------------------
     def halo_bytes(self, side: int) -> int:
         """Bytes of the strip side `side` (_lib.NORTH/SOUTH/EAST/WEST) sends."""

@@ -84,6 +84,18 @@ struct Handle {
     float *std_dq[ICAR_ADV_MAX_SPECIES] = {};
     float2 *std_sc[ICAR_ADV_MAX_SPECIES] = {};
     int std_par = 0;
+    // output / restart snapshot: FP32 device staging, pinned host staging,
+    // a copy stream, and what to copy out when it is complete
+    float *snap_dev = nullptr, *snap_host = nullptr;
+    size_t snap_cap = 0;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t snap_ready = nullptr, snap_done = nullptr;
+    bool snap_pending = false;
+    struct SnapItem {
+        float *host;
+        size_t off, n;
+    };
+    std::vector<SnapItem> snap_items;
     // profiling
     bool prof = false;
     std::vector<TimedLaunch> pending;
@@ -226,6 +238,12 @@ void free_all(Handle *h)
     (void)hipFree(h->d_filter);
     (void)hipFree(h->d_partial);
     (void)hipFree(h->d_dt);
+    if (h->copy_stream) (void)hipStreamSynchronize(h->copy_stream);
+    (void)hipFree(h->snap_dev);
+    if (h->snap_host) (void)hipHostFree(h->snap_host);
+    if (h->snap_ready) (void)hipEventDestroy(h->snap_ready);
+    if (h->snap_done) (void)hipEventDestroy(h->snap_done);
+    if (h->copy_stream) (void)hipStreamDestroy(h->copy_stream);
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     fast_batch_destroy(h->batch);
     fast_destroy(h->fast);
@@ -620,6 +638,18 @@ size_t strip_table(Handle *h, const Box &b, HaloStrips *hs, bool as_source)
     return bytes;
 }
 
+// Completes a pending snapshot into the host arrays.
+int finish_snapshot(Handle *h)
+{
+    if (!h->snap_pending) return 0;
+    h->snap_pending = false;
+    HIPCHK(h, hipEventSynchronize(h->snap_done));
+    for (const auto &it : h->snap_items)
+        memcpy(it.host, h->snap_host + it.off, it.n * sizeof(float));
+    h->snap_items.clear();
+    return 0;
+}
+
 // Options and bounds the standard scheme needs: orders 1/3/5, flux_corr 0/1,
 // and a halo wide enough for flux3's stencil (plus one cell for the limiter's
 // extended tile), as ICAR's grid gives it (domain_obj.f90:2979-2989).
@@ -737,8 +767,10 @@ int icar_adv_create(const icar_adv_grid *grid, const icar_adv_opts *opts, void *
 int icar_adv_destroy(void *handle)
 {
     if (!handle) return 0;
-    free_all(static_cast<Handle *>(handle));
-    return 0;
+    Handle *h = static_cast<Handle *>(handle);
+    const int rc = finish_snapshot(h);  // a pending snapshot reaches the host arrays first
+    free_all(h);
+    return rc;
 }
 
 int icar_adv_bind_field(void *handle, int field_id, float *host, const int lb[3], const int ub[3])
@@ -1292,6 +1324,105 @@ int icar_adv_halo_exchange(void *from, int side, void *to)
     return 0;
 }
 
+int icar_adv_enforce_limits(void *handle)
+{
+    Handle *h;
+    int rc = check_handle(handle, &h);
+    if (rc) return rc;
+    for (int s = 0; s < h->nspecies; s++) {
+        SpeciesSlot &sl = h->sp[s];
+        if (!sl.bound) continue;
+        const double es = sl.storage == ICAR_ADV_STORE_F32 ? 4.0 : 2.0;
+        timed(h, K_CLAMP, 2.0 * es * cells(h),
+              [&] { launch_clamp(h->stream, h->N, species_dev(h, s), sl.storage); });
+    }
+    HIPCHK(h, hipGetLastError());
+    return 0;
+}
+
+int icar_adv_snapshot(void *handle, unsigned long long mask)
+{
+    Handle *h;
+    int rc = check_handle(handle, &h);
+    if (rc) return rc;
+    rc = finish_snapshot(h);
+    if (rc) return rc;
+    // what goes out, in FP32 words
+    std::vector<Handle::SnapItem> items;
+    size_t total = 0;
+    for (int f = 0; f < ICAR_ADV_NFIELDS; f++) {
+        if (!(mask & ICAR_ADV_MASK_FIELD(f)) || !h->host_field[f]) continue;
+        items.push_back({h->host_field[f], total, field_elems(h, f)});
+        total += field_elems(h, f);
+    }
+    for (int s = 0; s < h->nspecies; s++) {
+        if (!(mask & ICAR_ADV_MASK_SPECIES(s)) || !h->sp[s].bound) continue;
+        items.push_back({h->sp[s].host, total, h->N});
+        total += h->N;
+    }
+    if (items.empty()) return 0;
+    if (total > h->snap_cap) {
+        (void)hipFree(h->snap_dev);
+        if (h->snap_host) (void)hipHostFree(h->snap_host);
+        h->snap_dev = nullptr;
+        h->snap_host = nullptr;
+        h->snap_cap = 0;
+        rc = dalloc(h, (void **)&h->snap_dev, total * sizeof(float));
+        if (rc) return rc;
+        if (hipHostMalloc((void **)&h->snap_host, total * sizeof(float), hipHostMallocDefault) !=
+            hipSuccess) {
+            h->snap_host = nullptr;
+            return set_err(h, ICAR_ADV_ENOMEM, "pinned host staging of %zu bytes failed",
+                           total * sizeof(float));
+        }
+        h->snap_cap = total;
+    }
+    if (!h->copy_stream) {
+        HIPCHK(h, hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
+        HIPCHK(h, hipEventCreateWithFlags(&h->snap_ready, hipEventDisableTiming));
+        HIPCHK(h, hipEventCreateWithFlags(&h->snap_done, hipEventDisableTiming));
+    }
+    // on the compute stream: the current state into the device staging
+    size_t m = 0;
+    for (int f = 0; f < ICAR_ADV_NFIELDS; f++) {
+        if (!(mask & ICAR_ADV_MASK_FIELD(f)) || !h->host_field[f]) continue;
+        const auto &it = items[m++];
+        HIPCHK(h, hipMemcpyAsync(h->snap_dev + it.off, h->dev_field[f], it.n * sizeof(float),
+                                 hipMemcpyDeviceToDevice, h->stream));
+    }
+    for (int s = 0; s < h->nspecies; s++) {
+        SpeciesSlot &sl = h->sp[s];
+        if (!(mask & ICAR_ADV_MASK_SPECIES(s)) || !sl.bound) continue;
+        const auto &it = items[m++];
+        timed(h, K_SNAPSHOT, 2.0 * 4 * cells(h), [&] {
+            if (sl.storage == ICAR_ADV_STORE_F32)
+                (void)hipMemcpyAsync(h->snap_dev + it.off, species_dev(h, s), it.n * sizeof(float),
+                                     hipMemcpyDeviceToDevice, h->stream);
+            else
+                launch_unpack(h->stream, h->N, species_dev(h, s), h->snap_dev + it.off, sl.storage,
+                              sl.pow2);
+        });
+    }
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipEventRecord(h->snap_ready, h->stream));
+    // on the copy stream, overlapping whatever the compute stream does next
+    HIPCHK(h, hipStreamWaitEvent(h->copy_stream, h->snap_ready, 0));
+    HIPCHK(h, hipMemcpyAsync(h->snap_host, h->snap_dev, total * sizeof(float),
+                             hipMemcpyDeviceToHost, h->copy_stream));
+    HIPCHK(h, hipEventRecord(h->snap_done, h->copy_stream));
+    h->snap_items = items;
+    h->snap_pending = true;
+    return 0;
+}
+
+int icar_adv_snapshot_wait(void *handle)
+{
+    Handle *h;
+    int rc = check_handle(handle, &h);
+    if (rc) return rc;
+    return finish_snapshot(h);
+}
+
 const char *icar_adv_last_error(void *handle)
 {
     if (!handle) return g_create_error.c_str();
@@ -1678,7 +1809,8 @@ const char *kernel_name(int id)
                                          "convert16",   "face_coefficients", "wind_balance",
                                          "dt_bound",    "forcing",    "std_wind",
                                          "std_stage",   "std_limiter_scale", "std_limiter_final",
-                                         "std_ring",    "halo_strip"};
+                                         "std_ring",    "halo_strip", "enforce_limits",
+                                         "snapshot"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "";
 }
 }  // namespace icaradv

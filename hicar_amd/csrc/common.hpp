@@ -45,6 +45,8 @@ enum KernelId {
     K_STD_FINAL,
     K_STD_RING,
     K_HALO,
+    K_CLAMP,
+    K_SNAPSHOT,
     K_COUNT
 };
 
@@ -113,6 +115,8 @@ struct HaloStrips {
 };
 // mode 0: a -> buffer; 1: buffer -> b; 2: a -> b
 void launch_strip(hipStream_t s, const HaloStrips &hs, int nz, int mode);
+// enforce_limits on n elements of one species at its storage kind
+void launch_clamp(hipStream_t s, size_t n, void *q, int storage);
 
 // ---- standard scheme, adv=1 (kernels_std.hip) ----
 // Geometry of the flux-form update: the tile (0-based memory offsets), the

@@ -14,8 +14,11 @@
 //                 order, so the result is the reference's bits).
 //   k_forcing     the boundary-forced branch of apply_forcing
 //                 (src/objects/domain_obj.f90:3493-3503) for one species.
+//   k_strip       the halo strips of the species for the halo exchange.
+//   k_clamp       enforce_limits (domain_obj.f90:3092-3150) for one species.
 // Every expression keeps the Fortran's evaluation order; this unit is
 // compiled with -ffp-contract=off.
+#include "../../include/icar_adv.h"
 #include "common.hpp"
 
 namespace icaradv {
@@ -267,6 +270,44 @@ void launch_strip(hipStream_t s, const HaloStrips &hs, int nz, int mode)
     long long blocks = (total + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_strip, dim3((unsigned)blocks), dim3(256), 0, s, hs, nz, per, mode);
+}
+
+}  // namespace icaradv
+
+namespace icaradv {
+
+namespace {
+
+// enforce_limits (domain_obj.f90:3092-3150, the data_3d branch): where a
+// species is below zero it becomes zero.  -0.0 and NaN are not below zero
+// and stay.  16-bit storage is judged on its bits: a set sign bit with a
+// non-zero magnitude that is not a NaN.
+__global__ void __launch_bounds__(256)
+    k_clamp(size_t n, void *__restrict__ q, int storage)
+{
+    for (size_t p = (size_t)blockIdx.x * 256 + threadIdx.x; p < n; p += (size_t)gridDim.x * 256) {
+        if (storage == ICAR_ADV_STORE_F32) {
+            float *f = static_cast<float *>(q);
+            const float v = f[p];
+            if (v < 0.0f) f[p] = 0.0f;
+        } else {
+            uint16_t *h = static_cast<uint16_t *>(q);
+            const uint16_t v = h[p];
+            const bool bf = storage == ICAR_ADV_STORE_BF16 || storage == ICAR_ADV_STORE_BF16_SR;
+            const uint16_t expm = bf ? 0x7f80u : 0x7c00u, manm = bf ? 0x007fu : 0x03ffu;
+            const bool nan = (v & expm) == expm && (v & manm) != 0;
+            if ((v & 0x8000u) && (v & 0x7fffu) && !nan) h[p] = 0;
+        }
+    }
+}
+
+}  // namespace
+
+void launch_clamp(hipStream_t s, size_t n, void *q, int storage)
+{
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_clamp, dim3((unsigned)blocks), dim3(256), 0, s, n, q, storage);
 }
 
 }  // namespace icaradv

@@ -75,7 +75,8 @@ __device__ __forceinline__ void put(float2 *p, off i, float2 v)
 // into chunks of kch levels (one per grid z index).
 struct Region {
     int i_lo, i_hi, j_lo, j_hi, kch;
-    int nbx, nby;  // blocks along i and j (set by march_grid)
+    int nbx, nby, nbz;  // blocks along i, j and k (set by with_blocks)
+    int sg;             // species per block (a group of the batch)
 };
 
 __device__ __forceinline__ float fmx(float a, float b) { return a > b ? a : b; }
@@ -180,6 +181,7 @@ constexpr int RB = 4;   // rows (waves) per block
 struct Lane {
     int i, j, fi;    // cell, and the column whose west face the lane supplies
     int k0, k1;      // the chunk of levels [k0, k1)
+    int s0, s1;      // the species [s0, s1) of the batch
     bool cell;       // the lane updates cell (i, j)
     off c0, u0;      // level 0 of column fi in cells and in the x-face array
 };
@@ -187,21 +189,26 @@ struct Lane {
 // Blocks are dealt round-robin over the 8 XCDs; renumber them so that each
 // XCD takes a contiguous range (neighbouring rows of tiles, whose y
 // stencils overlap, then share that XCD's L2).  The launch is 1-D.
-__device__ __forceinline__ void block_xyz(const Region &r, int &bx, int &by, int &bz)
+__device__ __forceinline__ void block_xyz(const Region &r, int &bx, int &by, int &bz, int &bg)
 {
     const unsigned L = blockIdx.x, B = gridDim.x;
     const unsigned q = B / 8, rem = B % 8, x = L % 8, slot = L / 8;
     const unsigned id = x < rem ? x * (q + 1) + slot : rem * (q + 1) + (x - rem) * q + slot;
-    bx = (int)(id % (unsigned)r.nbx);
-    by = (int)((id / (unsigned)r.nbx) % (unsigned)r.nby);
-    bz = (int)(id / ((unsigned)r.nbx * (unsigned)r.nby));
+    const unsigned per = (unsigned)r.nbx * (unsigned)r.nby * (unsigned)r.nbz;
+    const unsigned t = id % per;
+    bg = (int)(id / per);  // the species group is the slowest index
+    bx = (int)(t % (unsigned)r.nbx);
+    by = (int)((t / (unsigned)r.nbx) % (unsigned)r.nby);
+    bz = (int)(t / ((unsigned)r.nbx * (unsigned)r.nby));
 }
 
 __device__ __forceinline__ Lane lane_of(const Dims &d, const Region &r)
 {
     Lane l;
-    int bx, by, bz;
-    block_xyz(r, bx, by, bz);
+    int bx, by, bz, bg;
+    block_xyz(r, bx, by, bz, bg);
+    l.s0 = bg * r.sg;
+    l.s1 = l.s0 + r.sg;
     const int lane = threadIdx.x;
     l.i = r.i_lo + bx * CW + lane;
     l.j = r.j_lo + by * RB + (int)threadIdx.y;
@@ -287,132 +294,11 @@ __global__ void __launch_bounds__(256)
     Rdj[c] = f.dz[c] * f.jaco[c] * r0;  // dz*jaco*rho of :309
 }
 
-// flux3's face formula from an array of values, the positive-side cell at p
-template <int H, int N>
-__device__ __forceinline__ float face_a(float u, const float (&a)[N], int p)
-{
-    if constexpr (H == 1) {
-        return face_up(u, a[p - 1], a[p]);
-    } else if constexpr (H == 3) {
-        return face_o3(u, a[p - 2], a[p - 1], a[p], a[p + 1]);
-    } else {
-        return face_o5(u, a[p - 3], a[p - 2], a[p - 1], a[p], a[p + 1], a[p + 2]);
-    }
-}
-
-// What one level of a column reads, loaded one level ahead of its use.
-template <int H, int MODE>
-struct StageLevel {
-    static constexpr int R = (H + 1) / 2;
-    float uw, vs, vn, wt, rj, rdj;
-    float x[2 * R];      // qin(c-R .. c+R-1): the west face
-    float y[2 * R + 1];  // qin(c-R*sj .. c+R*sj): the south and north faces
-    float zn;            // qin(k+4), the next value of the z window
-    float q0, qw, qs, qn, qt;  // qold at c and its neighbours (dumb_q)
-};
-
-template <int H, int MODE>
-__device__ __forceinline__ void load_level(StageLevel<H, MODE> &L, const G<float> &qin,
-                                           const G<float> &qo, const G<float> &Um,
-                                           const G<float> &Vm, const G<float> &Wm,
-                                           const G<float> &Rj, const G<float> &Rdj, off c, off cu,
-                                           int k, int nz, off sj, off sz)
-{
-    constexpr int R = (H + 1) / 2;
-    L.uw = Um[cu];
-    L.vs = Vm[c];
-    L.vn = Vm[c + sj];
-    L.wt = Wm[c];
-    L.rj = Rj[c];
-    L.rdj = Rdj[c];
-    if (MODE <= 1) {
-#pragma unroll
-        for (int m = 0; m < 2 * R; m++) L.x[m] = qin[c + m - R];
-#pragma unroll
-        for (int m = 0; m < 2 * R + 1; m++) L.y[m] = qin[c + (off)(m - R) * sj];
-        L.zn = k + 4 < nz ? qin[c + 4 * sz] : 0.0f;
-        L.q0 = qo[c];
-    }
-    if (MODE >= 1) {
-        L.q0 = qo[c];
-        L.qw = qo[c - 1];
-        L.qs = qo[c - sj];
-        L.qn = qo[c + sj];
-        L.qt = k + 1 < nz ? qo[c + sz] : 0.0f;
-    }
-}
-
 // ---------------------------------------------------------------------------
 // one RK3 stage / Euler step without the limiter (MODE 0, tile), the same
 // plus the dumb_q half step on the tile and one ring (MODE 1, that region),
 // or dumb_q alone (MODE 2)
 // ---------------------------------------------------------------------------
-template <int H, int V, int MODE>
-__global__ void __launch_bounds__(64 * RB) k_std_stage(StdGeom g, float t, StdBatch b, Region r)
-{
-    constexpr int R = (H + 1) / 2;
-    const Dims d = g.d;
-    const Lane l = lane_of(d, r);
-    if (l.j > r.j_hi) return;  // a whole wave
-    const G<float> Um{g.Um}, Vm{g.Vm}, Wm{g.Wm}, Rj{g.Rj}, Rdj{g.Rdj};
-    const bool tile = l.cell && l.i >= g.its && l.i <= g.ite && l.j >= g.jts && l.j <= g.jte;
-    const off sj = (off)d.nx * d.nz, sz = d.nx, su = d.nx + 1;
-    for (int s = 0; s < b.n; s++) {
-        const G<float> qin{b.qin[s]}, qo{b.qold[s]};
-        float zw[6];  // qin(k-2 .. k+3) of the column
-        float fzb = 0.0f, hzb = 0.0f;  // through the face under the level (:252, :168)
-        if (MODE <= 1) {
-            float w7[7];
-            load_window7(w7, qin, l.c0, l.k0, d.nz, sz);
-            if (l.k0 > 0) fzb = face_under<V>(Wm[l.c0 + (off)(l.k0 - 1) * sz] * t, w7, l.k0, d.nz);
-#pragma unroll
-            for (int m = 0; m < 6; m++) zw[m] = w7[m + 1];
-        }
-        if (MODE >= 1 && l.k0 > 0) {
-            const off cb = l.c0 + (off)(l.k0 - 1) * sz;
-            hzb = half_up(Wm[cb], qo[cb], qo[cb + sz]);
-        }
-        StageLevel<H, MODE> cur;
-        load_level(cur, qin, qo, Um, Vm, Wm, Rj, Rdj, l.c0 + (off)l.k0 * sz,
-                   l.u0 + (off)l.k0 * su, l.k0, d.nz, sj, sz);
-#pragma unroll 1
-        for (int k = l.k0; k < l.k1; k++) {
-            const off c = l.c0 + (off)k * sz;
-            StageLevel<H, MODE> nxt;
-            if (k + 1 < l.k1)  // the same for the whole wave
-                load_level(nxt, qin, qo, Um, Vm, Wm, Rj, Rdj, c + sz, l.u0 + (off)(k + 1) * su,
-                           k + 1, d.nz, sj, sz);
-            if (MODE <= 1) {  // flux3 with the winds times t (advect.f90:285-287)
-                const float fxw = face_a<H>(cur.uw * t, cur.x, R);
-                const float fxe = __shfl_down(fxw, 1);
-                const float fys = face_a<H>(cur.vs * t, cur.y, R);
-                const float fyn = face_a<H>(cur.vn * t, cur.y, R + 1);
-                const float fzt = k == d.nz - 1 ? zw[2] * (cur.wt * t)  // :253
-                                                : face_vw<V>(cur.wt * t, zw, k + 1, d.nz);
-                if (tile)
-                    put(b.out[s], c,
-                        flux_update(cur.q0, fxw, fxe, fys, fyn, fzb, fzt, cur.rj, cur.rdj));
-                fzb = fzt;
-#pragma unroll
-                for (int m = 0; m < 5; m++) zw[m] = zw[m + 1];
-                zw[5] = cur.zn;
-            }
-            if (MODE >= 1) {  // dumb_q (adv_fluxcorr.f90:158-175)
-                const float q0 = cur.q0;
-                const float hxw = half_up(cur.uw, cur.qw, q0);
-                const float hxe = __shfl_down(hxw, 1);
-                const float hys = half_up(cur.vs, cur.qs, q0);
-                const float hyn = half_up(cur.vn, q0, cur.qn);
-                const float hzt = k == d.nz - 1 ? 0.5f * q0 * cur.wt : half_up(cur.wt, q0, cur.qt);
-                if (l.cell)
-                    put(b.dq[s], c, flux_update(q0, hxw, hxe, hys, hyn, hzb, hzt, cur.rj, cur.rdj));
-                hzb = hzt;
-            }
-            cur = nxt;
-        }
-    }
-}
-
 // vertical face kf (1..nz-1) whose upper cell is c, read from memory
 template <int V, class A, class I>
 __device__ __forceinline__ float face_vg(float w, const A &q, I c, int kf, int nz, I s)
@@ -423,11 +309,10 @@ __device__ __forceinline__ float face_vg(float w, const A &q, I c, int kf, int n
     return face_o3(w, q[c - 2 * s], q[c - s], q[c], q[c + s]);
 }
 
-// The same stage as k_std_stage, one thread per cell of the region (64
-// along i by 4 levels per block, a row per block column), every face
-// computed by each cell that uses it.  A stage has little work per face,
-// so the parallelism of this layout is worth more than the shared faces
-// the march gives the limiter kernels.
+// One thread per cell of the region (64 along i by 4 levels per block, a
+// row per block column), every face computed by each cell that uses it.  A
+// stage has little work per face, so the parallelism of this layout is
+// worth more than the shared faces a column march would give it.
 template <int H, int V, int MODE>
 __global__ void __launch_bounds__(256) k_std_stage_cell(StdGeom g, float t, StdBatch b, Region r)
 {
@@ -482,7 +367,7 @@ __global__ void __launch_bounds__(64 * RB) k_std_scale(StdGeom g, StdBatch b, Re
     if (l.j > r.j_hi) return;
     const G<float> Um{g.Um}, Vm{g.Vm}, Wm{g.Wm}, Rj{g.Rj}, Rdj{g.Rdj};
     const off sj = (off)d.nx * d.nz, sz = d.nx;
-    for (int s = 0; s < b.n; s++) {
+    for (int s = l.s0; s < min(l.s1, b.n); s++) {
         const G<float> qin{b.qin[s]}, qo{b.qold[s]}, dq{b.dq[s]};
         float zw[6], w7[7];
         load_window7(w7, qin, l.c0, l.k0, d.nz, sz);
@@ -588,7 +473,7 @@ __global__ void __launch_bounds__(64 * RB) k_std_final(StdGeom g, StdBatch b, Re
     if (l.j > r.j_hi) return;
     const G<float> Um{g.Um}, Vm{g.Vm}, Wm{g.Wm}, Rj{g.Rj}, Rdj{g.Rdj};
     const off sj = (off)d.nx * d.nz, sz = d.nx;
-    for (int s = 0; s < b.n; s++) {
+    for (int s = l.s0; s < min(l.s1, b.n); s++) {
         const G<float> qin{b.qin[s]}, qo{b.qold[s]}, dq{b.dq[s]};
         const G<float2> sc{b.sc[s]};
         float zw[6], w7[7];
@@ -693,25 +578,35 @@ int chunk_levels(int nz)
     return (nz + n - 1) / n;
 }
 
-Region with_blocks(Region r)
+// species per block of the limiter kernels: one species per block keeps
+// the blocks short, so that neighbouring tiles run in step and share their
+// stencil rows in L2 (measured 5% faster than a block looping over ten)
+constexpr int SPECIES_PER_BLOCK = 1;
+
+Region with_blocks(Region r, int nz, int nspecies)
 {
     r.nbx = (r.i_hi - r.i_lo + 1 + CW - 1) / CW;
     r.nby = (r.j_hi - r.j_lo + 1 + RB - 1) / RB;
+    r.nbz = (nz + r.kch - 1) / r.kch;
+    r.sg = std::min(nspecies, SPECIES_PER_BLOCK);
     return r;
 }
 
-dim3 march_grid(const Region &r, int nz)
+dim3 march_grid(const Region &r, int nspecies)
 {
-    return dim3((unsigned)r.nbx * r.nby * ((nz + r.kch - 1) / r.kch));
+    const unsigned groups = (unsigned)((nspecies + r.sg - 1) / r.sg);
+    return dim3((unsigned)r.nbx * r.nby * r.nbz * groups);
 }
 
-Region tile_of(const StdGeom &g)
+Region tile_of(const StdGeom &g, int nspecies)
 {
-    return with_blocks({g.its, g.ite, g.jts, g.jte, chunk_levels(g.d.nz), 0, 0});
+    return with_blocks({g.its, g.ite, g.jts, g.jte, chunk_levels(g.d.nz), 0, 0, 0, 0}, g.d.nz,
+                       nspecies);
 }
-Region ext_of(const StdGeom &g)
+Region ext_of(const StdGeom &g, int nspecies)
 {
-    return with_blocks({g.its - 1, g.ite + 1, g.jts - 1, g.jte + 1, chunk_levels(g.d.nz), 0, 0});
+    return with_blocks({g.its - 1, g.ite + 1, g.jts - 1, g.jte + 1, chunk_levels(g.d.nz), 0, 0, 0, 0},
+                       g.d.nz, nspecies);
 }
 
 template <int H, int V>
@@ -720,17 +615,17 @@ void launch_std_orders(hipStream_t s, int what, const StdGeom &g, float t, int m
 {
     const dim3 block(64, RB);
     if (what == 0) {
-        const Region r = mode == 0 ? tile_of(g) : ext_of(g);
+        const Region r = mode == 0 ? tile_of(g, b.n) : ext_of(g, b.n);
         const dim3 cg((r.i_hi - r.i_lo + 64) / 64, (g.d.nz + 3) / 4, r.j_hi - r.j_lo + 1);
         if (mode == 0) hipLaunchKernelGGL((k_std_stage_cell<H, V, 0>), cg, dim3(64, 4), 0, s, g, t, b, r);
         if (mode == 1) hipLaunchKernelGGL((k_std_stage_cell<H, V, 1>), cg, dim3(64, 4), 0, s, g, t, b, r);
         if (mode == 2) hipLaunchKernelGGL((k_std_stage_cell<H, V, 2>), cg, dim3(64, 4), 0, s, g, t, b, r);
     } else if (what == 1) {
-        const Region r = ext_of(g);
-        hipLaunchKernelGGL((k_std_scale<H, V>), march_grid(r, g.d.nz), block, 0, s, g, b, r);
+        const Region r = ext_of(g, b.n);
+        hipLaunchKernelGGL((k_std_scale<H, V>), march_grid(r, b.n), block, 0, s, g, b, r);
     } else {
-        const Region r = tile_of(g);
-        hipLaunchKernelGGL((k_std_final<H, V>), march_grid(r, g.d.nz), block, 0, s, g, b, r);
+        const Region r = tile_of(g, b.n);
+        hipLaunchKernelGGL((k_std_final<H, V>), march_grid(r, b.n), block, 0, s, g, b, r);
     }
 }
 

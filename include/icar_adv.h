@@ -240,6 +240,22 @@ int icar_adv_halo_pack(void *handle, int side, void *dev_buf, unsigned long long
 int icar_adv_halo_unpack(void *handle, int side, const void *dev_buf, unsigned long long bytes);
 int icar_adv_halo_exchange(void *from, int side, void *to);
 
+/* = domain%enforce_limits (domain_obj.f90:3092-3150, called in the last
+ * updates of a step, time_step.f90:386-389): every bound species below zero
+ * becomes zero on the device (-0.0 and NaN stay, as with `where (q < 0)`). */
+int icar_adv_enforce_limits(void *handle);
+
+/* Output and restart hand-off (output_obj.f90:202-230 writes the domain
+ * fields): icar_adv_snapshot copies the fields and species `mask` selects
+ * (the upload/download mask bits) as they are after the work queued so far,
+ * first into device staging on the handle's stream (a device copy; 16-bit
+ * species decoded to FP32), then into pinned host staging on a copy stream
+ * that overlaps the following steps.  icar_adv_snapshot_wait blocks until
+ * that copy is done and writes the values into the bound host arrays.  A
+ * second snapshot, or destroy, completes a pending one first. */
+int icar_adv_snapshot(void *handle, unsigned long long mask);
+int icar_adv_snapshot_wait(void *handle);
+
 int icar_adv_set_relax_filter(void *handle, int west, int east, int north, int south, int halo);
 int icar_adv_set_forcing(void *handle, int idx, const float *forcing_hi, const float *dqdt);
 int icar_adv_apply_forcing(void *handle, double dt);
