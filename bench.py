@@ -132,6 +132,60 @@ def measure_std(nx: int, ny: int, steps: int, warmup: int):
                         for k, v in stats.items()}}
 
 
+def measure_host_handoff(case, steps: int, warmup: int):
+    """What keeping ICAR's CPU physics in the loop costs the step: the
+    species' upload and download (the minimal policy uploads what the CPU
+    physics changed, all species after mp, and downloads what advect
+    changed, all species, every step), and what an output snapshot every
+    step adds to the step (its copy out overlaps the following steps)."""
+    from hicar_amd.advection import MpdataAdvection
+    d = case.domain
+    nbytes = 4.0 * d.nx * d.ny * d.nz * len(d.species)
+    case.options.kernel_variant = 1
+    saved = dict(d.storage)
+    d.storage.clear()  # FP32 species, as the host holds them
+    st = MpdataAdvection(d, case.options)
+    out = {}
+    try:
+        st.run(case.dt, warmup)
+        st.sync()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            st.upload(fields=False, species=True)
+        t1 = time.perf_counter()
+        for _ in range(3):
+            st.download(species=True)
+        t2 = time.perf_counter()
+        up, down = (t1 - t0) / 3, (t2 - t1) / 3
+        out["species_bytes"] = nbytes
+        out["upload_ms"] = 1e3 * up
+        out["download_ms"] = 1e3 * down
+        out["upload_GBps"] = nbytes / up / 1e9
+        out["download_GBps"] = nbytes / down / 1e9
+        t0 = time.perf_counter()
+        st.run(case.dt, steps)
+        st.sync()
+        plain = (time.perf_counter() - t0) / steps
+        # one snapshot of every species, then the steps while it is copied out
+        # (a first one allocates and pins the staging)
+        st.snapshot(species=True)
+        st.snapshot_wait()
+        t0 = time.perf_counter()
+        st.snapshot(species=True)
+        st.run(case.dt, steps)
+        st.sync()
+        t1 = time.perf_counter()
+        st.snapshot_wait()
+        t2 = time.perf_counter()
+        out["step_ms"] = 1e3 * plain
+        out["step_ms_while_snapshot_in_flight"] = 1e3 * (t1 - t0) / steps
+        out["snapshot_to_host_ms"] = 1e3 * (t2 - t0)
+    finally:
+        st.close()
+        d.storage.update(saved)
+    return out
+
+
 def measure_wind_steps(case, reps: int = 10):
     """The steps next to advect on the device copies: balance_uvw and
     compute_dt on the workload's winds, and apply_forcing of every species
@@ -310,6 +364,14 @@ def main() -> int:
         ensemble = measure_ensemble(args.steps, args.warmup)
         log(f"  ensemble of 48: eager {ensemble['eager']['value'] / 1e9:.2f}, graph "
             f"{ensemble['graph']['value'] / 1e9:.2f} G cell-species/s")
+    handoff = None
+    if args.variant == "all":
+        handoff = measure_host_handoff(case, min(args.steps, 10), args.warmup)
+        log(f"  host hand-off: species upload {handoff['upload_ms']:.1f} ms "
+            f"({handoff['upload_GBps']:.1f} GB/s), download {handoff['download_ms']:.1f} ms "
+            f"({handoff['download_GBps']:.1f} GB/s); step {handoff['step_ms']:.2f} ms, "
+            f"{handoff['step_ms_while_snapshot_in_flight']:.2f} ms while a snapshot is copied "
+            f"out ({handoff['snapshot_to_host_ms']:.0f} ms to the host arrays)")
     std = None
     if args.variant == "all" and not args.no_std:
         log("standard scheme (h5/v3, WRF limiter, RK3), same domain: generating inputs ...")
@@ -376,6 +438,7 @@ def main() -> int:
         "launch_bound_256x64x20": small,
         "ensemble_48": ensemble,
         "standard_scheme_rk3": std,
+        "host_handoff": handoff,
         "wind_steps": wind,
     }
     if not args.no_cpu:

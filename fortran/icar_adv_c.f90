@@ -20,6 +20,7 @@ module icar_adv_c
     public :: icar_adv_balance_w, icar_adv_compute_dt, icar_adv_launch_config
     public :: icar_adv_set_relax_filter, icar_adv_set_forcing, icar_adv_apply_forcing
     public :: icar_adv_halo_bytes, icar_adv_halo_pack, icar_adv_halo_unpack, icar_adv_halo_exchange
+    public :: icar_adv_enforce_limits, icar_adv_snapshot, icar_adv_snapshot_wait
     public :: ICAR_ADV_NORTH, ICAR_ADV_SOUTH, ICAR_ADV_EAST, ICAR_ADV_WEST
     public :: ICAR_ADV_FIELD_U, ICAR_ADV_FIELD_V, ICAR_ADV_FIELD_W, ICAR_ADV_FIELD_DENSITY
     public :: ICAR_ADV_FIELD_JACOBIAN, ICAR_ADV_FIELD_JACOBIAN_U, ICAR_ADV_FIELD_JACOBIAN_V
@@ -234,6 +235,25 @@ module icar_adv_c
             import :: c_int, c_ptr
             type(c_ptr), value :: from, to
             integer(c_int), value :: side
+        end function
+
+        !> domain%enforce_limits (domain_obj.f90:3092-3150) on the device species
+        integer(c_int) function icar_adv_enforce_limits(handle) bind(C, name="icar_adv_enforce_limits")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+        end function
+
+        !> output/restart hand-off: start copying the masked fields and species
+        !! out (overlaps the following steps), then complete into the bound arrays
+        integer(c_int) function icar_adv_snapshot(handle, mask) bind(C, name="icar_adv_snapshot")
+            import :: c_int, c_ptr, c_long_long
+            type(c_ptr), value :: handle
+            integer(c_long_long), value :: mask
+        end function
+
+        integer(c_int) function icar_adv_snapshot_wait(handle) bind(C, name="icar_adv_snapshot_wait")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
         end function
 
         integer(c_int) function icar_adv_launch_config(handle, corr_cfg, ksplit) &

@@ -1369,8 +1369,9 @@ int icar_adv_snapshot(void *handle, unsigned long long mask)
         h->snap_cap = 0;
         rc = dalloc(h, (void **)&h->snap_dev, total * sizeof(float));
         if (rc) return rc;
-        if (hipHostMalloc((void **)&h->snap_host, total * sizeof(float), hipHostMallocDefault) !=
-            hipSuccess) {
+        // non-coherent: cacheable for the CPU copy into the host arrays
+        if (hipHostMalloc((void **)&h->snap_host, total * sizeof(float),
+                          hipHostMallocNonCoherent) != hipSuccess) {
             h->snap_host = nullptr;
             return set_err(h, ICAR_ADV_ENOMEM, "pinned host staging of %zu bytes failed",
                            total * sizeof(float));
