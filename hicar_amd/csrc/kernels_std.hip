@@ -14,9 +14,9 @@
 //   k_std_final   the normalised, re-merged fluxes (:129-150) and the update.
 //   k_std_ring    the cells outside the tile, which every call leaves at the
 //                 base state (qfluxes = qold, :298).
-// The stage, scale and final kernels march up columns (see Lane below) and
-// loop over a batch of species; each x and z face flux is computed once and
-// shared, the y faces by both rows.  Every evaluation is the same IEEE FP32
+// The stage, scale and final kernels march up the columns of LDS-staged
+// tiles, one species per block, and compute each face flux once (see Tile
+// below).  Every evaluation is the same IEEE FP32
 // expression in the Fortran's order (this unit is compiled with
 // -ffp-contract=off), so the results are the bits of a strict build of the
 // reference.  Fortran MAX/MIN of two reals are the selects a > b ? a : b and
@@ -72,11 +72,11 @@ __device__ __forceinline__ void put(float2 *p, off i, float2 v)
 }
 
 // A block of cells [i_lo, i_hi] x [j_lo, j_hi] (0-based), all levels, cut
-// into chunks of kch levels (one per grid z index).
+// into tiles and chunks of kch levels.
 struct Region {
     int i_lo, i_hi, j_lo, j_hi, kch;
-    int nbx, nby, nbz;  // blocks along i, j and k (set by with_blocks)
-    int sg;             // species per block (a group of the batch)
+    int nbx, nby, nbz;  // tiles along i and j, chunks along k
+    int ns;             // species of the launch, one per block
 };
 
 __device__ __forceinline__ float fmx(float a, float b) { return a > b ? a : b; }
@@ -167,58 +167,15 @@ __device__ __forceinline__ float up2(float u, float qm, float qp, float dm, floa
     return f;
 }
 
-// The kernels march up the columns of a region, a chunk of levels per
-// thread: a wave holds 64 columns along i and updates the first 63; each
-// lane computes the fluxes through the x face on the west side of its column
-// and takes the east face from the next lane (lane 63 only supplies that
-// face).  The face under level k is the one computed above level k-1,
-// carried in registers, so each z face is computed once too, apart from the
-// face under the first level of a chunk; the y faces are computed by both
-// rows.
-constexpr int CW = 63;  // columns a wave updates
-constexpr int RB = 4;   // rows (waves) per block
-
-struct Lane {
-    int i, j, fi;    // cell, and the column whose west face the lane supplies
-    int k0, k1;      // the chunk of levels [k0, k1)
-    int s0, s1;      // the species [s0, s1) of the batch
-    bool cell;       // the lane updates cell (i, j)
-    off c0, u0;      // level 0 of column fi in cells and in the x-face array
-};
+constexpr int CW = 63;  // columns a wave updates (lane 63 supplies a face only)
 
 // Blocks are dealt round-robin over the 8 XCDs; renumber them so that each
-// XCD takes a contiguous range (neighbouring rows of tiles, whose y
-// stencils overlap, then share that XCD's L2).  The launch is 1-D.
-__device__ __forceinline__ void block_xyz(const Region &r, int &bx, int &by, int &bz, int &bg)
+// XCD takes a contiguous range of the block order.  The launch is 1-D.
+__device__ __forceinline__ unsigned xcd_block()
 {
     const unsigned L = blockIdx.x, B = gridDim.x;
     const unsigned q = B / 8, rem = B % 8, x = L % 8, slot = L / 8;
-    const unsigned id = x < rem ? x * (q + 1) + slot : rem * (q + 1) + (x - rem) * q + slot;
-    const unsigned per = (unsigned)r.nbx * (unsigned)r.nby * (unsigned)r.nbz;
-    const unsigned t = id % per;
-    bg = (int)(id / per);  // the species group is the slowest index
-    bx = (int)(t % (unsigned)r.nbx);
-    by = (int)((t / (unsigned)r.nbx) % (unsigned)r.nby);
-    bz = (int)(t / ((unsigned)r.nbx * (unsigned)r.nby));
-}
-
-__device__ __forceinline__ Lane lane_of(const Dims &d, const Region &r)
-{
-    Lane l;
-    int bx, by, bz, bg;
-    block_xyz(r, bx, by, bz, bg);
-    l.s0 = bg * r.sg;
-    l.s1 = l.s0 + r.sg;
-    const int lane = threadIdx.x;
-    l.i = r.i_lo + bx * CW + lane;
-    l.j = r.j_lo + by * RB + (int)threadIdx.y;
-    l.fi = min(l.i, r.i_hi + 1);
-    l.cell = lane < CW && l.i <= r.i_hi;
-    l.k0 = bz * r.kch;
-    l.k1 = min(d.nz, l.k0 + r.kch);
-    l.c0 = (off)l.fi + (off)d.nx * (off)d.nz * l.j;
-    l.u0 = (off)l.fi + (off)(d.nx + 1) * (off)d.nz * l.j;
-    return l;
+    return x < rem ? x * (q + 1) + slot : rem * (q + 1) + (x - rem) * q + slot;
 }
 
 // q(k0-3 .. k0+3) of a column (zero outside it): w[0..5] is the stencil of
@@ -295,236 +252,512 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
-// one RK3 stage / Euler step without the limiter (MODE 0, tile), the same
-// plus the dumb_q half step on the tile and one ring (MODE 1, that region),
-// or dumb_q alone (MODE 2)
+// Column marches over tiles staged in LDS
 // ---------------------------------------------------------------------------
-// vertical face kf (1..nz-1) whose upper cell is c, read from memory
-template <int V, class A, class I>
-__device__ __forceinline__ float face_vg(float w, const A &q, I c, int kf, int nz, I s)
+// A block covers CW columns by TY rows of the region, one wave per row, plus
+// a wave for row TY that supplies the y faces north of the tile's last row.
+// Its lanes march up a chunk of levels together.  At each level the block
+// stages the planes its stencils read in LDS: each lane its own cell, and a
+// few waves the halo around the tile.  Then every face is computed once:
+//   x  on the west side of each lane's column (the east face is the next
+//      lane's; lane 63 only supplies it),
+//   y  on the south side of each row (the north face is the next row's,
+//      passed through LDS after a second barrier),
+//   z  above each level (the face under a level is carried in registers).
+// Blocks take one species each, the species fastest in the block order, so
+// that the species of a tile run together on one XCD and read the tile's
+// winds and divisors from its L2.
+template <int H, int TY>
+struct Tile {
+    static constexpr int HH = (H + 1) / 2;         // stencil half width of flux3
+    static constexpr int QR = TY + 2 * HH;         // qin plane: rows j0-HH .. j0+TY+HH-1
+    static constexpr int QC = CW + 2 * HH;         //   columns i0-HH .. i0+62+HH
+    static constexpr int PR = TY + 2;              // qold/dumb_q/scales: rows j0-1 .. j0+TY
+    static constexpr int PC = 65;                  //   columns i0-1 .. i0+63
+    static constexpr int NXH = (2 * HH - 1) * TY;  // qin cells west and east of the tile rows
+    static constexpr int NT = 64 * (TY + 1);       // threads
+    // halo roles: waves 0 .. 2HH-2 stage a qin row below or above the tile,
+    // wave TY-2 the column west of the tile and wave TY-1 the row south of
+    // it in the other planes, wave TY the qin cells west and east of the rows
+    static_assert(TY - 2 > 2 * HH - 2 && NXH <= 128, "tile rows");
+};
+
+struct TLane {
+    int L, w;         // lane; wave = row of the tile (w == TY: y faces only)
+    int i, j;         // this lane's column and row
+    int k0, k1, s;    // levels [k0, k1) and the species
+    bool cell;        // the lane updates (i, j)
+    off cb, ub;       // (i, j) at level 0 in cells and in the x-face array
+    int pq, pp;       // (i, j) in the qin plane and in the other planes
+    off hq[2];        // halo: up to two qin cells (level 0 offsets)
+    int hqd[2];       //   their place in the qin plane, -1 if none
+    bool hqv[2];      //   inside the domain
+    off ho;           // halo: one cell of the other planes
+    int hod;          //   its place, -1 if none
+    bool hov;
+};
+
+template <int H, int TY>
+__device__ __forceinline__ TLane tile_lane(const Dims &d, const Region &r)
 {
-    if (V == 1 || kf == 1 || kf == nz - 1) return face_up(w, q[c - s], q[c]);
-    if (V == 5 && kf >= 3 && kf <= nz - 3)
-        return face_o5(w, q[c - 3 * s], q[c - 2 * s], q[c - s], q[c], q[c + s], q[c + 2 * s]);
-    return face_o3(w, q[c - 2 * s], q[c - s], q[c], q[c + s]);
+    using T = Tile<H, TY>;
+    constexpr int HH = T::HH;
+    TLane l;
+    unsigned t = xcd_block();
+    l.s = (int)(t % (unsigned)r.ns);
+    t /= (unsigned)r.ns;
+    const int bz = (int)(t % (unsigned)r.nbz);
+    t /= (unsigned)r.nbz;
+    const int bx = (int)(t % (unsigned)r.nbx), by = (int)(t / (unsigned)r.nbx);
+    l.L = (int)threadIdx.x & 63;
+    l.w = (int)threadIdx.x >> 6;
+    const int i0 = r.i_lo + bx * CW, j0 = r.j_lo + by * TY;
+    l.i = i0 + l.L;
+    l.j = j0 + l.w;
+    l.k0 = bz * r.kch;
+    l.k1 = min(d.nz, l.k0 + r.kch);
+    l.cell = l.L < CW && l.w < TY && l.i <= r.i_hi && l.j <= r.j_hi;
+    const bool dom = l.i < d.nx && l.j < d.ny;
+    const off sj = (off)d.nx * (off)d.nz;
+    // a lane outside the domain reads cell 0: its values reach only faces
+    // that no updated cell uses
+    l.cb = dom ? (off)l.i + sj * (off)l.j : 0u;
+    l.ub = dom ? (off)l.i + (off)(d.nx + 1) * (off)d.nz * (off)l.j : 0u;
+    l.pq = (HH + l.w) * T::QC + HH + l.L;
+    l.pp = (1 + l.w) * T::PC + 1 + l.L;
+    l.hqd[0] = l.hqd[1] = -1;
+    l.hq[0] = l.hq[1] = 0u;
+    l.hqv[0] = l.hqv[1] = false;
+    l.hod = -1;
+    l.ho = 0u;
+    l.hov = false;
+    auto at = [&](int ii, int jj, bool &v) -> off {
+        v = ii >= 0 && ii < d.nx && jj >= 0 && jj < d.ny;
+        return v ? (off)ii + sj * (off)jj : 0u;
+    };
+    if (l.w < 2 * HH - 1) {  // a qin row below (e < HH) or above the tile
+        const int e = l.w;
+        const int pr = e < HH ? e : TY + HH + 1 + (e - HH);
+        l.hq[0] = at(l.i, j0 - HH + pr, l.hqv[0]);
+        l.hqd[0] = pr * T::QC + HH + l.L;
+    } else if (l.w == TY - 2) {  // column i0-1 of the rows j0 .. j0+TY-1
+        if (l.L < TY) {
+            l.ho = at(i0 - 1, j0 + l.L, l.hov);
+            l.hod = (1 + l.L) * T::PC;
+        }
+    } else if (l.w == TY - 1) {  // row j0-1 of the columns i0 .. i0+63
+        l.ho = at(l.i, j0 - 1, l.hov);
+        l.hod = 1 + l.L;
+    } else if (l.w == TY) {  // qin columns i0-HH .. i0-1 and i0+64 .. i0+62+HH
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int e = l.L + 64 * u;
+            if (e < T::NXH) {
+                const int rr = e / (2 * HH - 1), cc = e % (2 * HH - 1);
+                const int q = cc < HH ? cc : 64 + cc;
+                l.hq[u] = at(i0 - HH + q, j0 + rr, l.hqv[u]);
+                l.hqd[u] = (HH + rr) * T::QC + q;
+            }
+        }
+    }
+    return l;
 }
 
-// One thread per cell of the region (64 along i by 4 levels per block, a
-// row per block column), every face computed by each cell that uses it.  A
-// stage has little work per face, so the parallelism of this layout is
-// worth more than the shared faces a column march would give it.
-template <int H, int V, int MODE>
-__global__ void __launch_bounds__(256) k_std_stage_cell(StdGeom g, float t, StdBatch b, Region r)
+// the halo values of level k: qin (Q) and the first NP of qold, dumb_q and
+// the scales
+struct THalo {
+    float q[2], o, dq;
+    float2 sc;
+};
+
+template <bool Q, int NP>
+__device__ __forceinline__ THalo tile_halo(const TLane &l, G<float> qin, G<float> qo, G<float> dq,
+                                           G<float2> sc, int k, off sz)
 {
+    THalo h;
+    const off kk = (off)k * sz;
+#pragma unroll
+    for (int u = 0; u < 2; u++) h.q[u] = Q && l.hqv[u] ? qin[l.hq[u] + kk] : 0.0f;
+    h.o = h.dq = 0.0f;
+    h.sc = make_float2(0.0f, 0.0f);
+    if (NP >= 1 && l.hov) {
+        h.o = qo[l.ho + kk];
+        if (NP >= 2) h.dq = dq[l.ho + kk];
+        if (NP >= 3) h.sc = sc[l.ho + kk];
+    }
+    return h;
+}
+
+template <bool Q, int NP>
+__device__ __forceinline__ void stage_halo(const TLane &l, const THalo &h, float *sq, float *so,
+                                           float *sd, float2 *ss)
+{
+    if (Q) {
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+            if (l.hqd[u] >= 0) sq[l.hqd[u]] = h.q[u];
+    }
+    if (NP >= 1 && l.hod >= 0) {
+        so[l.hod] = h.o;
+        if (NP >= 2) sd[l.hod] = h.dq;
+        if (NP >= 3) ss[l.hod] = h.sc;
+    }
+}
+
+// the winds and divisors of a cell at one level (the last wave needs only
+// the south face's wind)
+struct TWind {
+    float uw, vs, wt, rj, rdj;
+};
+
+__device__ __forceinline__ TWind tile_wind(const TLane &l, const StdGeom &g, int k, bool row)
+{
+    const G<float> Um{g.Um}, Vm{g.Vm}, Wm{g.Wm}, Rj{g.Rj}, Rdj{g.Rdj};
+    const off c = l.cb + (off)k * (off)g.d.nx;
+    TWind f;
+    f.vs = Vm[c];
+    f.uw = f.wt = 0.0f;
+    f.rj = f.rdj = 1.0f;
+    if (row) {
+        f.uw = Um[l.ub + (off)k * (off)(g.d.nx + 1)];
+        f.wt = Wm[c];
+        f.rj = Rj[c];
+        f.rdj = Rdj[c];
+    }
+    return f;
+}
+
+// Each iteration stages level k from registers, then issues the loads of
+// what the next iteration stages and computes with (the halo of level k+1,
+// the winds of level k+1, the lane's own values of level k+2), so that no
+// load is waited for in the iteration that issues it.
+
+// One RK3 stage / Euler step over tiles: flux3 of the stage state with the
+// winds times t applied to the base state (MODE 0 on the tile; MODE 1 the
+// same plus the dumb_q half step on the tile and one ring; MODE 2 dumb_q
+// alone on that region)
+template <int H, int V, int MODE, int TY>
+__global__ void __launch_bounds__(64 * (TY + 1))
+    k_std_stage(StdGeom g, float t, StdBatch b, Region r)
+{
+    using T = Tile<H, TY>;
+    constexpr bool FL = MODE <= 1, DQ = MODE >= 1;
+    __shared__ float sq[FL ? T::QR * T::QC : 1];
+    __shared__ float so[DQ ? T::PR * T::PC : 1];
+    __shared__ float sy[(TY + 1) * 64], sh[DQ ? (TY + 1) * 64 : 1];
     const Dims d = g.d;
-    const int i = r.i_lo + (int)blockIdx.x * 64 + (int)threadIdx.x;
-    const int k = (int)blockIdx.y * 4 + (int)threadIdx.y;
-    const int j = r.j_lo + (int)blockIdx.z;
-    if (i > r.i_hi || k >= d.nz) return;
-    const float *__restrict__ Um = g.Um, *__restrict__ Vm = g.Vm, *__restrict__ Wm = g.Wm;
-    const float *__restrict__ Rj = g.Rj, *__restrict__ Rdj = g.Rdj;
-    const bool tile = i >= g.its && i <= g.ite && j >= g.jts && j <= g.jte;
-    const soff sj = (soff)d.nx * d.nz, sz = d.nx;
-    const soff c = (soff)i + (soff)d.nx * ((soff)k + (soff)d.nz * j);
-    const soff fu = (soff)i + (soff)(d.nx + 1) * ((soff)k + (soff)d.nz * j);
-    const float uw = Um[fu], ue = Um[fu + 1], vs = Vm[c], vn = Vm[c + sj];
-    const float wb = k > 0 ? Wm[c - sz] : 0.0f, wt = Wm[c];
-    const float rj = Rj[c], rdj = Rdj[c];
-    for (int s = 0; s < b.n; s++) {
-        const float *__restrict__ qin = b.qin[s];
-        const float *__restrict__ qo = b.qold[s];
-        const float q0 = qo[c];
-        if (MODE <= 1 && tile) {  // flux3 with the winds times t (advect.f90:285-287)
-            const float fxw = face_h<H>(uw * t, qin, c, 1);
-            const float fxe = face_h<H>(ue * t, qin, c + 1, 1);
-            const float fys = face_h<H>(vs * t, qin, c, sj);
-            const float fyn = face_h<H>(vn * t, qin, c + sj, sj);
-            const float fzb = k == 0 ? 0.0f : face_vg<V>(wb * t, qin, c, k, d.nz, sz);  // :252
-            const float fzt = k == d.nz - 1 ? qin[c] * (wt * t)                          // :253
-                                            : face_vg<V>(wt * t, qin, c + sz, k + 1, d.nz, sz);
-            b.out[s][c] = flux_update(q0, fxw, fxe, fys, fyn, fzb, fzt, rj, rdj);
+    const TLane l = tile_lane<H, TY>(d, r);
+    const bool row = l.w < TY;
+    const bool in_tile = l.i >= g.its && l.i <= g.ite && l.j >= g.jts && l.j <= g.jte;
+    const bool put_fl = FL && l.cell && (MODE == 0 || in_tile);
+    const G<float> Wm{g.Wm};
+    const G<float> qin{b.qin[l.s]}, qo{b.qold[l.s]};
+    const G<float2> nosc{nullptr};
+    const off sz = d.nx;
+    float zw[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, w7[7];
+    if (FL) {
+        load_window7(w7, qin, l.cb, l.k0, d.nz, sz);
+#pragma unroll
+        for (int m = 0; m < 6; m++) zw[m] = w7[m + 1];
+    }
+    const off ck = l.cb + (off)l.k0 * sz;
+    float o0 = qo[ck], ot = l.k0 + 1 < d.nz ? qo[ck + sz] : 0.0f;  // qold here and above
+    float fzb = 0.0f, hzb = 0.0f;  // the faces under the level
+    if (row && l.k0 > 0) {
+        const float wb = Wm[ck - sz];
+        if (FL) fzb = face_under<V>(wb * t, w7, l.k0, d.nz);
+        if (DQ) hzb = half_up(wb, qo[ck - sz], o0);
+    }
+    THalo h = tile_halo<FL, DQ ? 1 : 0>(l, qin, qo, qo, nosc, l.k0, sz);
+    TWind f = tile_wind(l, g, l.k0, row);
+#pragma unroll 1
+    for (int k = l.k0; k < l.k1; k++) {
+        const off c = l.cb + (off)k * sz;
+        const bool top = k == d.nz - 1;
+        if (FL) sq[l.pq] = zw[2];
+        if (DQ) so[l.pp] = o0;
+        stage_halo<FL, DQ ? 1 : 0>(l, h, sq, so, nullptr, nullptr);
+        const float on = k + 2 < d.nz ? qo[c + 2 * sz] : 0.0f;
+        TWind fn = f;
+        if (k + 1 < l.k1) {
+            h = tile_halo<FL, DQ ? 1 : 0>(l, qin, qo, qo, nosc, k + 1, sz);
+            fn = tile_wind(l, g, k + 1, row);
         }
-        if (MODE >= 1) {  // dumb_q (adv_fluxcorr.f90:158-175)
-            const float hxw = half_up(uw, qo[c - 1], q0);
-            const float hxe = half_up(ue, q0, qo[c + 1]);
-            const float hys = half_up(vs, qo[c - sj], q0);
-            const float hyn = half_up(vn, q0, qo[c + sj]);
-            const float hzb = k == 0 ? 0.0f : half_up(wb, qo[c - sz], q0);
-            const float hzt = k == d.nz - 1 ? 0.5f * q0 * wt : half_up(wt, q0, qo[c + sz]);
-            b.dq[s][c] = flux_update(q0, hxw, hxe, hys, hyn, hzb, hzt, rj, rdj);
+        __syncthreads();
+        // flux3 with the winds times t (advect.f90:285-287) and the first
+        // upwind half step (adv_fluxcorr.f90:158-175) through the west face,
+        // the face above the level and the south face
+        float fxw = 0.0f, fzt = 0.0f, hxw = 0.0f, hzt = 0.0f, fys = 0.0f, hys = 0.0f;
+        if (row) {
+            if (FL) {
+                fxw = face_h<H>(f.uw * t, sq, l.pq, 1);
+                fzt = top ? zw[2] * (f.wt * t) : face_vw<V>(f.wt * t, zw, k + 1, d.nz);  // :253
+            }
+            if (DQ) {
+                hxw = half_up(f.uw, so[l.pp - 1], o0);
+                hzt = top ? 0.5f * o0 * f.wt : half_up(f.wt, o0, ot);
+            }
+        }
+        const float fxe = FL ? __shfl_down(fxw, 1) : 0.0f;
+        const float hxe = DQ ? __shfl_down(hxw, 1) : 0.0f;
+        if (FL) {
+            fys = face_h<H>(f.vs * t, sq, l.pq, T::QC);
+            sy[l.w * 64 + l.L] = fys;
+        }
+        if (DQ) {
+            hys = half_up(f.vs, so[l.pp - T::PC], o0);
+            sh[l.w * 64 + l.L] = hys;
+        }
+        __syncthreads();
+        const int n = (l.w + 1) * 64 + l.L;
+        if (put_fl) put(b.out[l.s], c, flux_update(o0, fxw, fxe, fys, sy[n], fzb, fzt, f.rj, f.rdj));
+        if (DQ && l.cell)
+            put(b.dq[l.s], c, flux_update(o0, hxw, hxe, hys, sh[n], hzb, hzt, f.rj, f.rdj));
+        fzb = fzt;
+        hzb = hzt;
+        o0 = ot;
+        ot = on;
+        f = fn;
+        if (FL) {
+#pragma unroll
+            for (int m = 0; m < 5; m++) zw[m] = zw[m + 1];
+            zw[5] = k + 4 < d.nz ? qin[c + 4 * sz] : 0.0f;
         }
     }
 }
 
-// ---------------------------------------------------------------------------
-// WRF_flux_corr up to scale_in / scale_out, on the tile plus one ring
-// ---------------------------------------------------------------------------
-template <int H, int V>
-__global__ void __launch_bounds__(64 * RB) k_std_scale(StdGeom g, StdBatch b, Region r)
+// WRF_flux_corr up to scale_in / scale_out (the region is the tile plus one
+// ring of cells)
+template <int H, int V, int TY>
+__global__ void __launch_bounds__(64 * (TY + 1)) k_std_scale(StdGeom g, StdBatch b, Region r)
 {
+    using T = Tile<H, TY>;
+    __shared__ float sq[T::QR * T::QC];
+    __shared__ float so[T::PR * T::PC], sd[T::PR * T::PC];
+    __shared__ float syu[(TY + 1) * 64], syh[(TY + 1) * 64], syv[(TY + 1) * 64];
     const Dims d = g.d;
-    const Lane l = lane_of(d, r);
-    if (l.j > r.j_hi) return;
-    const G<float> Um{g.Um}, Vm{g.Vm}, Wm{g.Wm}, Rj{g.Rj}, Rdj{g.Rdj};
-    const off sj = (off)d.nx * d.nz, sz = d.nx;
-    for (int s = l.s0; s < min(l.s1, b.n); s++) {
-        const G<float> qin{b.qin[s]}, qo{b.qold[s]}, dq{b.dq[s]};
-        float zw[6], w7[7];
-        load_window7(w7, qin, l.c0, l.k0, d.nz, sz);
+    const TLane l = tile_lane<H, TY>(d, r);
+    const bool row = l.w < TY;
+    const G<float> Wm{g.Wm};
+    const G<float> qin{b.qin[l.s]}, qo{b.qold[l.s]}, dq{b.dq[l.s]};
+    const G<float2> nosc{nullptr};
+    const off sz = d.nx;
+    float zw[6], w7[7];
+    load_window7(w7, qin, l.cb, l.k0, d.nz, sz);
 #pragma unroll
-        for (int m = 0; m < 6; m++) zw[m] = w7[m + 1];
-        const off ck = l.c0 + (off)l.k0 * sz;
-        float ob = 0.0f, o0 = qo[ck], d0 = dq[ck];  // qold below, qold and dumb_q here
-        float uzb = 0.0f, hzb = 0.0f, wb = 0.0f;    // the face under the level
-        if (l.k0 > 0) {
-            wb = Wm[ck - sz];
-            ob = qo[ck - sz];
-            uzb = up2(wb, ob, o0, dq[ck - sz], d0);
-            hzb = face_under<V>(wb, w7, l.k0, d.nz);
-        }
+    for (int m = 0; m < 6; m++) zw[m] = w7[m + 1];
+    const off ck = l.cb + (off)l.k0 * sz;
+    // qold below, here and above; dumb_q here and above
+    float ob = 0.0f, o0 = qo[ck], d0 = dq[ck], ot = 0.0f, dt_ = 0.0f;
+    if (l.k0 + 1 < d.nz) {
+        ot = qo[ck + sz];
+        dt_ = dq[ck + sz];
+    }
+    float uzb = 0.0f, hzb = 0.0f, wb = 0.0f;  // the face under the level
+    if (row && l.k0 > 0) {
+        wb = Wm[ck - sz];
+        ob = qo[ck - sz];
+        uzb = up2(wb, ob, o0, dq[ck - sz], d0);
+        hzb = face_under<V>(wb, w7, l.k0, d.nz);
+    }
+    THalo h = tile_halo<true, 2>(l, qin, qo, dq, nosc, l.k0, sz);
+    TWind f = tile_wind(l, g, l.k0, row);
 #pragma unroll 1
-        for (int k = l.k0; k < l.k1; k++) {
-            const off c = l.c0 + (off)k * sz;
-            const bool top = k == d.nz - 1;
-            const float ot = top ? 0.0f : qo[c + sz], dt_ = top ? 0.0f : dq[c + sz];
-            const float uw = Um[l.u0 + (off)k * (d.nx + 1)];
-            const float vs = Vm[c], vn = Vm[c + sj], wt = Wm[c];
-            const float rj = Rj[c], rdj = Rdj[c];
-            const float qw = qo[c - 1], qs = qo[c - sj], qn = qo[c + sj];
-            // the upwind fluxes of both half steps and flux3 with t_factor 1
-            const float uxw = up2(uw, qw, o0, dq[c - 1], d0);
-            const float hxw = face_h<H>(uw, qin, c, 1);
-            const float uxe = __shfl_down(uxw, 1), hxe = __shfl_down(hxw, 1);
-            const float ue = __shfl_down(uw, 1), qe = __shfl_down(o0, 1);
-            const float uys = up2(vs, qs, o0, dq[c - sj], d0);
-            const float uyn = up2(vn, o0, qn, d0, dq[c + sj]);
-            const float hys = face_h<H>(vs, qin, c, sj);
-            const float hyn = face_h<H>(vn, qin, c + sj, sj);
-            float uzt;
+    for (int k = l.k0; k < l.k1; k++) {
+        const off c = l.cb + (off)k * sz;
+        const bool top = k == d.nz - 1;
+        sq[l.pq] = zw[2];
+        so[l.pp] = o0;
+        sd[l.pp] = d0;
+        stage_halo<true, 2>(l, h, sq, so, sd, nullptr);
+        float on = 0.0f, dn = 0.0f;
+        if (k + 2 < d.nz) {
+            on = qo[c + 2 * sz];
+            dn = dq[c + 2 * sz];
+        }
+        TWind fn = f;
+        if (k + 1 < l.k1) {
+            h = tile_halo<true, 2>(l, qin, qo, dq, nosc, k + 1, sz);
+            fn = tile_wind(l, g, k + 1, row);
+        }
+        __syncthreads();
+        // the upwind fluxes of both half steps and flux3 with t_factor 1
+        float uxw = 0.0f, hxw = 0.0f, qw = 0.0f, qn = 0.0f, uzt = 0.0f, hzt = 0.0f;
+        if (row) {
+            qw = so[l.pp - 1];
+            qn = so[l.pp + T::PC];
+            uxw = up2(f.uw, qw, o0, sd[l.pp - 1], d0);
+            hxw = face_h<H>(f.uw, sq, l.pq, 1);
             if (top) {  // :169, :184
-                uzt = 0.5f * o0 * wt;
-                uzt = uzt + 0.5f * d0 * wt;
+                uzt = 0.5f * o0 * f.wt;
+                uzt = uzt + 0.5f * d0 * f.wt;
             } else {
-                uzt = up2(wt, o0, ot, d0, dt_);
+                uzt = up2(f.wt, o0, ot, d0, dt_);
             }
-            const float hzt = top ? zw[2] * wt : face_vw<V>(wt, zw, k + 1, d.nz);
-            if (l.cell) {
-                // :75-78 the upwind-only value
-                const float temp = flux_update(o0, uxw, uxe, uys, uyn, uzb, uzt, rj, rdj);
-                // :80-85 merged fluxes
-                const float mxw = hxw - uxw, mxe = hxe - uxe, mys = hys - uys, myn = hyn - uyn,
-                            mzb = hzb - uzb, mzt = hzt - uzt;
-                // :88-96
-                float fin = -((fmn(0.0f, mxe) - fmx(0.0f, mxw)) + (fmn(0.0f, myn) - fmx(0.0f, mys))) / rj;
-                fin = fin - (fmn(0.0f, mzt) - fmx(0.0f, mzb)) / rdj;
-                float fout = ((fmx(0.0f, mxe) - fmn(0.0f, mxw)) + (fmx(0.0f, myn) - fmn(0.0f, mys))) / rj;
-                fout = fout + (fmx(0.0f, mzt) - fmn(0.0f, mzb)) / rdj;
-                // :99-121 bounds from the upwind neighbours (unscaled winds)
-                float qmax = o0, qmin = o0;
-                if (uw > 0.0f) {
-                    qmax = fmx(qw, qmax);
-                    qmin = fmn(qw, qmin);
-                } else if (ue < 0.0f) {
-                    qmax = fmx(qe, qmax);
-                    qmin = fmn(qe, qmin);
-                }
-                if (vs > 0.0f) {
-                    qmax = fmx(qs, qmax);
-                    qmin = fmn(qs, qmin);
-                } else if (vn < 0.0f) {
-                    qmax = fmx(qn, qmax);
-                    qmin = fmn(qn, qmin);
-                }
-                if (k > 0 && wb > 0.0f) {
-                    qmax = fmx(ob, qmax);
-                    qmin = fmn(ob, qmin);
-                }
-                if (!top && wt < 0.0f) {
-                    qmax = fmx(ot, qmax);
-                    qmin = fmn(ot, qmin);
-                }
-                // :124-125
-                float si = 1.0f, so = 1.0f;
-                if (fin > (qmax - temp)) si = fmx(0.0f, (qmax - temp) / (fin + 1.E-15f));
-                if (fout > (temp - qmin)) so = fmx(0.0f, (temp - qmin) / (fout + 1.E-15f));
-                put(b.sc[s], c, make_float2(si, so));
-            }
-            uzb = uzt;
-            hzb = hzt;
-            wb = wt;
-            ob = o0;
-            o0 = ot;
-            d0 = dt_;
-#pragma unroll
-            for (int m = 0; m < 5; m++) zw[m] = zw[m + 1];
-            zw[5] = k + 4 < d.nz ? qin[c + 4 * sz] : 0.0f;
+            hzt = top ? zw[2] * f.wt : face_vw<V>(f.wt, zw, k + 1, d.nz);
         }
+        const float qs = so[l.pp - T::PC];
+        const float uxe = __shfl_down(uxw, 1), hxe = __shfl_down(hxw, 1);
+        const float ue = __shfl_down(f.uw, 1), qe = __shfl_down(o0, 1);
+        const float uys = up2(f.vs, qs, o0, sd[l.pp - T::PC], d0);
+        const float hys = face_h<H>(f.vs, sq, l.pq, T::QC);
+        syu[l.w * 64 + l.L] = uys;
+        syh[l.w * 64 + l.L] = hys;
+        syv[l.w * 64 + l.L] = f.vs;
+        __syncthreads();
+        if (l.cell) {
+            const int n = (l.w + 1) * 64 + l.L;
+            const float uyn = syu[n], hyn = syh[n], vn = syv[n];
+            // :75-78 the upwind-only value
+            const float temp = flux_update(o0, uxw, uxe, uys, uyn, uzb, uzt, f.rj, f.rdj);
+            // :80-85 merged fluxes
+            const float mxw = hxw - uxw, mxe = hxe - uxe, mys = hys - uys, myn = hyn - uyn,
+                        mzb = hzb - uzb, mzt = hzt - uzt;
+            // :88-96
+            float fin = -((fmn(0.0f, mxe) - fmx(0.0f, mxw)) + (fmn(0.0f, myn) - fmx(0.0f, mys))) / f.rj;
+            fin = fin - (fmn(0.0f, mzt) - fmx(0.0f, mzb)) / f.rdj;
+            float fout = ((fmx(0.0f, mxe) - fmn(0.0f, mxw)) + (fmx(0.0f, myn) - fmn(0.0f, mys))) / f.rj;
+            fout = fout + (fmx(0.0f, mzt) - fmn(0.0f, mzb)) / f.rdj;
+            // :99-121 bounds from the upwind neighbours (unscaled winds)
+            float qmax = o0, qmin = o0;
+            if (f.uw > 0.0f) {
+                qmax = fmx(qw, qmax);
+                qmin = fmn(qw, qmin);
+            } else if (ue < 0.0f) {
+                qmax = fmx(qe, qmax);
+                qmin = fmn(qe, qmin);
+            }
+            if (f.vs > 0.0f) {
+                qmax = fmx(qs, qmax);
+                qmin = fmn(qs, qmin);
+            } else if (vn < 0.0f) {
+                qmax = fmx(qn, qmax);
+                qmin = fmn(qn, qmin);
+            }
+            if (k > 0 && wb > 0.0f) {
+                qmax = fmx(ob, qmax);
+                qmin = fmn(ob, qmin);
+            }
+            if (!top && f.wt < 0.0f) {
+                qmax = fmx(ot, qmax);
+                qmin = fmn(ot, qmin);
+            }
+            // :124-125
+            float si = 1.0f, so_ = 1.0f;
+            if (fin > (qmax - temp)) si = fmx(0.0f, (qmax - temp) / (fin + 1.E-15f));
+            if (fout > (temp - qmin)) so_ = fmx(0.0f, (temp - qmin) / (fout + 1.E-15f));
+            put(b.sc[l.s], c, make_float2(si, so_));
+        }
+        uzb = uzt;
+        hzb = hzt;
+        wb = f.wt;
+        ob = o0;
+        o0 = ot;
+        d0 = dt_;
+        ot = on;
+        dt_ = dn;
+        f = fn;
+#pragma unroll
+        for (int m = 0; m < 5; m++) zw[m] = zw[m + 1];
+        zw[5] = k + 4 < d.nz ? qin[c + 4 * sz] : 0.0f;
     }
 }
 
-// ---------------------------------------------------------------------------
 // the limited fluxes and the final update, on the tile
-// ---------------------------------------------------------------------------
-template <int H, int V>
-__global__ void __launch_bounds__(64 * RB) k_std_final(StdGeom g, StdBatch b, Region r)
+template <int H, int V, int TY>
+__global__ void __launch_bounds__(64 * (TY + 1)) k_std_final(StdGeom g, StdBatch b, Region r)
 {
+    using T = Tile<H, TY>;
+    __shared__ float sq[T::QR * T::QC];
+    __shared__ float so[T::PR * T::PC], sd[T::PR * T::PC];
+    __shared__ float2 ss[T::PR * T::PC];
+    __shared__ float sy[(TY + 1) * 64];
     const Dims d = g.d;
-    const Lane l = lane_of(d, r);
-    if (l.j > r.j_hi) return;
-    const G<float> Um{g.Um}, Vm{g.Vm}, Wm{g.Wm}, Rj{g.Rj}, Rdj{g.Rdj};
-    const off sj = (off)d.nx * d.nz, sz = d.nx;
-    for (int s = l.s0; s < min(l.s1, b.n); s++) {
-        const G<float> qin{b.qin[s]}, qo{b.qold[s]}, dq{b.dq[s]};
-        const G<float2> sc{b.sc[s]};
-        float zw[6], w7[7];
-        load_window7(w7, qin, l.c0, l.k0, d.nz, sz);
+    const TLane l = tile_lane<H, TY>(d, r);
+    const bool row = l.w < TY;
+    const G<float> Wm{g.Wm};
+    const G<float> qin{b.qin[l.s]}, qo{b.qold[l.s]}, dq{b.dq[l.s]};
+    const G<float2> sc{b.sc[l.s]};
+    const off sz = d.nx;
+    float zw[6], w7[7];
+    load_window7(w7, qin, l.cb, l.k0, d.nz, sz);
 #pragma unroll
-        for (int m = 0; m < 6; m++) zw[m] = w7[m + 1];
-        const off ck = l.c0 + (off)l.k0 * sz;
-        float o0 = qo[ck], d0 = dq[ck];
-        float2 s0 = sc[ck];  // this cell's scales
-        float Fzb = 0.0f;    // through the face under the level
-        if (l.k0 > 0) {
-            const float wb = Wm[ck - sz];
-            const float uzb = up2(wb, qo[ck - sz], o0, dq[ck - sz], d0);
-            Fzb = limit_face(face_under<V>(wb, w7, l.k0, d.nz) - uzb, s0, sc[ck - sz]) + uzb;
-        }
+    for (int m = 0; m < 6; m++) zw[m] = w7[m + 1];
+    const off ck = l.cb + (off)l.k0 * sz;
+    // qold, dumb_q and the scales of this cell and the one above
+    float o0 = qo[ck], d0 = dq[ck], ot = 0.0f, dt_ = 0.0f;
+    float2 s0 = sc[ck], st = make_float2(0.0f, 0.0f);
+    if (l.k0 + 1 < d.nz) {
+        ot = qo[ck + sz];
+        dt_ = dq[ck + sz];
+        st = sc[ck + sz];
+    }
+    float Fzb = 0.0f;  // through the face under the level
+    if (row && l.k0 > 0) {
+        const float wb = Wm[ck - sz];
+        const float uzb = up2(wb, qo[ck - sz], o0, dq[ck - sz], d0);
+        Fzb = limit_face(face_under<V>(wb, w7, l.k0, d.nz) - uzb, s0, sc[ck - sz]) + uzb;
+    }
+    THalo h = tile_halo<true, 3>(l, qin, qo, dq, sc, l.k0, sz);
+    TWind f = tile_wind(l, g, l.k0, row);
 #pragma unroll 1
-        for (int k = l.k0; k < l.k1; k++) {
-            const off c = l.c0 + (off)k * sz;
-            const bool top = k == d.nz - 1;
-            const float ot = top ? 0.0f : qo[c + sz], dt_ = top ? 0.0f : dq[c + sz];
-            const float2 st = top ? make_float2(0.0f, 0.0f) : sc[c + sz];
-            const float uw = Um[l.u0 + (off)k * (d.nx + 1)];
-            const float vs = Vm[c], vn = Vm[c + sj], wt = Wm[c];
-            const float rj = Rj[c], rdj = Rdj[c];
-            // x: the west face, normalised with this cell (positive side) and
-            // the one to the west; the east face is the next lane's
-            const float uxw = up2(uw, qo[c - 1], o0, dq[c - 1], d0);
-            const float Fxw = limit_face(face_h<H>(uw, qin, c, 1) - uxw, s0, sc[c - 1]) + uxw;
-            const float Fxe = __shfl_down(Fxw, 1);
-            const float uys = up2(vs, qo[c - sj], o0, dq[c - sj], d0);
-            const float uyn = up2(vn, o0, qo[c + sj], d0, dq[c + sj]);
-            const float Fys = limit_face(face_h<H>(vs, qin, c, sj) - uys, s0, sc[c - sj]) + uys;
-            const float Fyn = limit_face(face_h<H>(vn, qin, c + sj, sj) - uyn, sc[c + sj], s0) + uyn;
-            float uzt, Fzt;
-            if (top) {  // merged but not limited (:139-143 stop at kme)
-                uzt = 0.5f * o0 * wt;
-                uzt = uzt + 0.5f * d0 * wt;
-                Fzt = (zw[2] * wt - uzt) + uzt;
-            } else {
-                uzt = up2(wt, o0, ot, d0, dt_);
-                Fzt = limit_face(face_vw<V>(wt, zw, k + 1, d.nz) - uzt, st, s0) + uzt;
-            }
-            if (l.cell) put(b.out[s], c, flux_update(o0, Fxw, Fxe, Fys, Fyn, Fzb, Fzt, rj, rdj));
-            Fzb = Fzt;
-            o0 = ot;
-            d0 = dt_;
-            s0 = st;
-#pragma unroll
-            for (int m = 0; m < 5; m++) zw[m] = zw[m + 1];
-            zw[5] = k + 4 < d.nz ? qin[c + 4 * sz] : 0.0f;
+    for (int k = l.k0; k < l.k1; k++) {
+        const off c = l.cb + (off)k * sz;
+        const bool top = k == d.nz - 1;
+        sq[l.pq] = zw[2];
+        so[l.pp] = o0;
+        sd[l.pp] = d0;
+        ss[l.pp] = s0;
+        stage_halo<true, 3>(l, h, sq, so, sd, ss);
+        float on = 0.0f, dn = 0.0f;
+        float2 sn = make_float2(0.0f, 0.0f);
+        if (k + 2 < d.nz) {
+            on = qo[c + 2 * sz];
+            dn = dq[c + 2 * sz];
+            sn = sc[c + 2 * sz];
         }
+        TWind fn = f;
+        if (k + 1 < l.k1) {
+            h = tile_halo<true, 3>(l, qin, qo, dq, sc, k + 1, sz);
+            fn = tile_wind(l, g, k + 1, row);
+        }
+        __syncthreads();
+        // x: the west face, normalised with this cell (positive side) and
+        // the one to the west; z: the face above the level
+        float Fxw = 0.0f, Fzt = 0.0f;
+        if (row) {
+            const float uxw = up2(f.uw, so[l.pp - 1], o0, sd[l.pp - 1], d0);
+            Fxw = limit_face(face_h<H>(f.uw, sq, l.pq, 1) - uxw, s0, ss[l.pp - 1]) + uxw;
+            if (top) {  // merged but not limited (:139-143 stop at kme)
+                float uzt = 0.5f * o0 * f.wt;
+                uzt = uzt + 0.5f * d0 * f.wt;
+                Fzt = (zw[2] * f.wt - uzt) + uzt;
+            } else {
+                const float uzt = up2(f.wt, o0, ot, d0, dt_);
+                Fzt = limit_face(face_vw<V>(f.wt, zw, k + 1, d.nz) - uzt, st, s0) + uzt;
+            }
+        }
+        const float Fxe = __shfl_down(Fxw, 1);
+        // y: the south face, the north one is the next row's
+        const int ps = l.pp - T::PC;
+        const float uys = up2(f.vs, so[ps], o0, sd[ps], d0);
+        const float Fys = limit_face(face_h<H>(f.vs, sq, l.pq, T::QC) - uys, s0, ss[ps]) + uys;
+        sy[l.w * 64 + l.L] = Fys;
+        __syncthreads();
+        if (l.cell)
+            put(b.out[l.s], c,
+                flux_update(o0, Fxw, Fxe, Fys, sy[(l.w + 1) * 64 + l.L], Fzb, Fzt, f.rj, f.rdj));
+        Fzb = Fzt;
+        o0 = ot;
+        d0 = dt_;
+        s0 = st;
+        ot = on;
+        dt_ = dn;
+        st = sn;
+        f = fn;
+#pragma unroll
+        for (int m = 0; m < 5; m++) zw[m] = zw[m + 1];
+        zw[5] = k + 4 < d.nz ? qin[c + 4 * sz] : 0.0f;
     }
 }
 
@@ -569,63 +802,60 @@ __global__ void __launch_bounds__(256) k_std_ring(StdGeom g, StdBatch b, int wit
 
 dim3 cells_grid(const Dims &d) { return dim3((d.nx + 63) / 64, (d.nz + 3) / 4, d.ny); }
 
-// levels per chunk: at most 16, in equal chunks (two chunks of 15 for 30
-// levels measured best on the 1024x1024x30 domain: 8-level chunks cost 3%,
-// whole columns 7%, both from L2 reuse between the rows of tiles in flight)
-int chunk_levels(int nz)
-{
-    const int n = std::max(1, (nz + 15) / 16);
-    return (nz + n - 1) / n;
-}
+// Tile rows: 7 rows and the y-face wave make 8 waves per block, which pack
+// a CU exactly at the kernels' occupancy (6 waves per SIMD for the limiter
+// kernels: 3 blocks, 8 for the stages: 4 blocks).  9- and 16-wave blocks
+// measured 20-35% slower for leaving wave slots empty.
+constexpr int TILE_ROWS = 7;
 
-// species per block of the limiter kernels: one species per block keeps
-// the blocks short, so that neighbouring tiles run in step and share their
-// stencil rows in L2 (measured 5% faster than a block looping over ten)
-constexpr int SPECIES_PER_BLOCK = 1;
-
-Region with_blocks(Region r, int nz, int nspecies)
+// The region in tiles, one species per block.  Whole columns, unless the
+// grid would have too few blocks to fill the GPU; then the columns are cut
+// into chunks (each chunk recomputes the face under its first level).
+Region tiles(int i_lo, int i_hi, int j_lo, int j_hi, int nz, int nspecies)
 {
-    r.nbx = (r.i_hi - r.i_lo + 1 + CW - 1) / CW;
-    r.nby = (r.j_hi - r.j_lo + 1 + RB - 1) / RB;
+    Region r{};
+    r.i_lo = i_lo;
+    r.i_hi = i_hi;
+    r.j_lo = j_lo;
+    r.j_hi = j_hi;
+    r.nbx = (i_hi - i_lo + 1 + CW - 1) / CW;
+    r.nby = (j_hi - j_lo + 1 + TILE_ROWS - 1) / TILE_ROWS;
+    r.ns = nspecies;
+    r.kch = nz;
+    while (r.kch > 4 && (long long)r.nbx * r.nby * r.ns * ((nz + r.kch - 1) / r.kch) < 1024)
+        r.kch = (r.kch + 1) / 2;
     r.nbz = (nz + r.kch - 1) / r.kch;
-    r.sg = std::min(nspecies, SPECIES_PER_BLOCK);
     return r;
-}
-
-dim3 march_grid(const Region &r, int nspecies)
-{
-    const unsigned groups = (unsigned)((nspecies + r.sg - 1) / r.sg);
-    return dim3((unsigned)r.nbx * r.nby * r.nbz * groups);
 }
 
 Region tile_of(const StdGeom &g, int nspecies)
 {
-    return with_blocks({g.its, g.ite, g.jts, g.jte, chunk_levels(g.d.nz), 0, 0, 0, 0}, g.d.nz,
-                       nspecies);
+    return tiles(g.its, g.ite, g.jts, g.jte, g.d.nz, nspecies);
 }
+
 Region ext_of(const StdGeom &g, int nspecies)
 {
-    return with_blocks({g.its - 1, g.ite + 1, g.jts - 1, g.jte + 1, chunk_levels(g.d.nz), 0, 0, 0, 0},
-                       g.d.nz, nspecies);
+    return tiles(g.its - 1, g.ite + 1, g.jts - 1, g.jte + 1, g.d.nz, nspecies);
 }
+
+dim3 tile_grid(const Region &r) { return dim3((unsigned)r.nbx * r.nby * r.nbz * r.ns); }
 
 template <int H, int V>
 void launch_std_orders(hipStream_t s, int what, const StdGeom &g, float t, int mode,
                        const StdBatch &b)
 {
-    const dim3 block(64, RB);
+    const dim3 tb(Tile<H, TILE_ROWS>::NT);
     if (what == 0) {
         const Region r = mode == 0 ? tile_of(g, b.n) : ext_of(g, b.n);
-        const dim3 cg((r.i_hi - r.i_lo + 64) / 64, (g.d.nz + 3) / 4, r.j_hi - r.j_lo + 1);
-        if (mode == 0) hipLaunchKernelGGL((k_std_stage_cell<H, V, 0>), cg, dim3(64, 4), 0, s, g, t, b, r);
-        if (mode == 1) hipLaunchKernelGGL((k_std_stage_cell<H, V, 1>), cg, dim3(64, 4), 0, s, g, t, b, r);
-        if (mode == 2) hipLaunchKernelGGL((k_std_stage_cell<H, V, 2>), cg, dim3(64, 4), 0, s, g, t, b, r);
+        if (mode == 0) hipLaunchKernelGGL((k_std_stage<H, V, 0, TILE_ROWS>), tile_grid(r), tb, 0, s, g, t, b, r);
+        if (mode == 1) hipLaunchKernelGGL((k_std_stage<H, V, 1, TILE_ROWS>), tile_grid(r), tb, 0, s, g, t, b, r);
+        if (mode == 2) hipLaunchKernelGGL((k_std_stage<H, V, 2, TILE_ROWS>), tile_grid(r), tb, 0, s, g, t, b, r);
     } else if (what == 1) {
         const Region r = ext_of(g, b.n);
-        hipLaunchKernelGGL((k_std_scale<H, V>), march_grid(r, b.n), block, 0, s, g, b, r);
+        hipLaunchKernelGGL((k_std_scale<H, V, TILE_ROWS>), tile_grid(r), tb, 0, s, g, b, r);
     } else {
         const Region r = tile_of(g, b.n);
-        hipLaunchKernelGGL((k_std_final<H, V>), march_grid(r, b.n), block, 0, s, g, b, r);
+        hipLaunchKernelGGL((k_std_final<H, V, TILE_ROWS>), tile_grid(r), tb, 0, s, g, b, r);
     }
 }
 

@@ -65,14 +65,34 @@ def test_std_matches_oracle(oracle32, h, v, fc, rk3):
 @pytest.mark.parametrize("h,v,fc,rk3", [(5, 3, 1, True), (5, 5, 1, True), (3, 3, 0, True),
                                         (1, 3, 1, False)])
 def test_std_tall_column_chunks_match_oracle(oracle32, h, v, fc, rk3):
-    """21 levels: the limiter kernels march each column in two chunks, the
-    second starting from the face under its first level."""
+    """21 levels on a grid of few tiles: the kernels cut each column into
+    chunks of 3 levels (to give the GPU more blocks), each chunk starting
+    from the face under its first level."""
     case = std_case(h, v, fc, rk3, nx=36, ny=28, nz=21)
     d = case.domain
     ref = oracle_steps(oracle32, case, 2)
     st = Advection(d, case.options)
     try:
         st.run(case.dt, 2)
+        st.download()
+    finally:
+        st.close()
+    for n in ref:
+        np.testing.assert_array_equal(d.species[n], ref[n], err_msg=n)
+
+
+@pytest.mark.parametrize("h,v,fc,rk3", [(5, 3, 1, True), (3, 5, 0, True)])
+def test_std_whole_columns_match_oracle(oracle32, h, v, fc, rk3):
+    """Enough tiles and species (8 x 16 tiles of 63 x 7 cells, 16 species)
+    that every block marches a whole column of 9 levels, as on full-size
+    domains."""
+    names = [f"t{k:02d}" for k in range(16)]
+    case = std_case(h, v, fc, rk3, nx=63 * 8 + 8, ny=7 * 16 + 8, nz=9, names=names)
+    d = case.domain
+    ref = oracle_steps(oracle32, case, 1)
+    st = Advection(d, case.options)
+    try:
+        st.advect(case.dt)
         st.download()
     finally:
         st.close()
