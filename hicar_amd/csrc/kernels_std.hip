@@ -82,6 +82,24 @@ struct Region {
 __device__ __forceinline__ float fmx(float a, float b) { return a > b ? a : b; }
 __device__ __forceinline__ float fmn(float a, float b) { return a < b ? a : b; }
 
+// x / C for the constants C = 12 and 60 of flux3, with R = 1/C rounded: the
+// quotient estimate q = x*R, its remainder x - q*C (exact in one FMA) and
+// q + remainder*R rounded once.  Compared with the IEEE division over every
+// float for both constants, this returns the IEEE quotient for each x with
+// |x| >= 2^-124 and for +-0 (whose sign copysign restores); smaller x (whose
+// quotient is near or below the subnormal range), infinities and NaNs take
+// the IEEE division, a branch the lanes of a wave skip unless one needs it.
+template <int C>
+__device__ __forceinline__ float div_const(float x)
+{
+    constexpr float R = 1.0f / (float)C;
+    const uint32_t a = __float_as_uint(x) & 0x7fffffffu;
+    if (__builtin_expect(a - 1u < 0x017fffffu || a >= 0x7f800000u, 0)) return x / (float)C;
+    const float q = x * R;
+    const float e = __builtin_fmaf(-q, (float)C, x);
+    return __builtin_copysignf(__builtin_fmaf(e, R, q), x);
+}
+
 // flux3's face formulas (advect.f90:150-178): m3..p2 are q at offsets -3..+2
 // from the cell on the positive side of the face.
 __device__ __forceinline__ float face_up(float u, float m1, float p0)
@@ -92,8 +110,8 @@ __device__ __forceinline__ float face_up(float u, float m1, float p0)
 __device__ __forceinline__ float face_o3(float u, float m2, float m1, float p0, float p1)
 {
     float f = 7.0f * (p0 + m1) - (p1 + m2);
-    f = (u * f) / 12.0f;
-    f = f - (fabsf(u) / 12.0f) * (3.0f * (p0 - m1) - (p1 - m2));
+    f = div_const<12>(u * f);
+    f = f - div_const<12>(fabsf(u)) * (3.0f * (p0 - m1) - (p1 - m2));
     return f;
 }
 
@@ -101,8 +119,8 @@ __device__ __forceinline__ float face_o5(float u, float m3, float m2, float m1, 
                                          float p1, float p2)
 {
     float f = 37.0f * (p0 + m1) - 8.0f * (p1 + m2) + (p2 + m3);
-    f = (u * f) / 60.0f;
-    f = f - (fabsf(u) / 60.0f) * (10.0f * (p0 - m1) - 5.0f * (p1 - m2) + (p2 - m3));
+    f = div_const<60>(u * f);
+    f = f - div_const<60>(fabsf(u)) * (10.0f * (p0 - m1) - 5.0f * (p1 - m2) + (p2 - m3));
     return f;
 }
 

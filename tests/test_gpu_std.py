@@ -100,6 +100,33 @@ def test_std_whole_columns_match_oracle(oracle32, h, v, fc, rk3):
         np.testing.assert_array_equal(d.species[n], ref[n], err_msg=n)
 
 
+@pytest.mark.parametrize("h,v,fc,rk3", [(5, 3, 1, True), (3, 5, 0, True), (5, 5, 1, False)])
+def test_std_tiny_and_subnormal_values_match_oracle(oracle32, h, v, fc, rk3):
+    """Species of magnitude 1e-28 down to 1e-40: face fluxes u*f fall around
+    and below 2^-124 ~ 4.7e-38, where flux3's divisions by 12 and 60 leave
+    their three-operation path for the IEEE division, and quotients and
+    values are subnormal.  Bitwise against the oracle (IEEE FP32 with
+    subnormals)."""
+    case = std_case(h, v, fc, rk3, names=("a", "b", "c", "d"))
+    d = case.domain
+    for a, scale in zip(d.species.values(), (1e-28, 1e-33, 1e-36, 1e-39)):
+        a *= scale
+    init = {n: a.copy() for n, a in d.species.items()}
+    assert (0 < init["d"]).any() and (init["d"][init["d"] > 0] < 1.17549435e-38).any()
+    ref = oracle_steps(oracle32, case, 2)
+    st = Advection(d, case.options)
+    try:
+        st.advect(case.dt)
+        st.advect(case.dt)
+        st.download()
+    finally:
+        st.close()
+    for n in ref:
+        np.testing.assert_array_equal(d.species[n].view(np.uint32), ref[n].view(np.uint32),
+                                      err_msg=n)
+        assert not np.array_equal(ref[n], init[n]), n
+
+
 def test_std_host_mirror_keeps_positivity_below_top():
     """adv_init + advect (the host mirror) with the namelist settings.  The
     limiter leaves the model-top flux alone, so the top level can go
