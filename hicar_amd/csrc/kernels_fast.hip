@@ -622,51 +622,45 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
         // x faces and divergence at L2 (own rows), y faces at L2 (staged
         // rows), z faces at L1 (own rows)
         float kx[OWN][4], kz[OWN][5], ky[SR][4], kg[OWN][2];
+        // The loads are unconditional, from a valid cell: a lane outside the
+        // domain reads cell 0 and a level out of range the nearest one.  The
+        // values are used only where the lane and level are in range (the
+        // guards of the phases below), and no defaults are kept per lane.
         auto load_coef = [&]() {
+            const unsigned l2 = (unsigned)max(L2, 0), l1 = (unsigned)max(0, min(L1, nz - 2));
 #pragma unroll
             for (int t = 0; t < OWN; t++) {
-#pragma unroll
-                for (int e = 0; e < 4; e++) kx[t][e] = 0.0f;
-#pragma unroll
-                for (int e = 0; e < 5; e++) kz[t][e] = 0.0f;
-                kg[t][0] = kg[t][1] = 1.0f;
-                if (own_in[t]) {
-                    if (L2 >= 0) {
-                        const unsigned c = own_base[t] + sk * (unsigned)L2;
-                        gcfloat *v4 = gx + 4 * (size_t)c;
-                        const float4 v = make_float4(v4[0], v4[1], v4[2], v4[3]);
-                        kx[t][0] = v.x;
-                        kx[t][1] = v.y;
-                        kx[t][2] = v.z;
-                        kx[t][3] = v.w;
-                        kg[t][0] = Gp[c];
-                        kg[t][1] = Gzp[c];
-                    }
-                    if (L1 >= 0 && L1 <= nz - 2) {
-                        const unsigned c = own_base[t] + sk * (unsigned)L1;
-                        gcfloat *v4 = gz + 4 * (size_t)c;
-                        const float4 v = make_float4(v4[0], v4[1], v4[2], v4[3]);
-                        kz[t][0] = v.x;
-                        kz[t][1] = v.y;
-                        kz[t][2] = v.z;
-                        kz[t][3] = v.w;
-                        kz[t][4] = dzp[c];
-                    }
+                {
+                    const unsigned c = own_base[t] + sk * l2;
+                    gcfloat *v4 = gx + 4 * (size_t)c;
+                    const float4 v = make_float4(v4[0], v4[1], v4[2], v4[3]);
+                    kx[t][0] = v.x;
+                    kx[t][1] = v.y;
+                    kx[t][2] = v.z;
+                    kx[t][3] = v.w;
+                    kg[t][0] = Gp[c];
+                    kg[t][1] = Gzp[c];
+                }
+                {
+                    const unsigned c = own_base[t] + sk * l1;
+                    gcfloat *v4 = gz + 4 * (size_t)c;
+                    const float4 v = make_float4(v4[0], v4[1], v4[2], v4[3]);
+                    kz[t][0] = v.x;
+                    kz[t][1] = v.y;
+                    kz[t][2] = v.z;
+                    kz[t][3] = v.w;
+                    kz[t][4] = dzp[c];
                 }
             }
 #pragma unroll
             for (int r = 0; r < SR; r++) {
-#pragma unroll
-                for (int e = 0; e < 4; e++) ky[r][e] = 0.0f;
-                if (row_in[r] && L2 >= 0) {
-                    const unsigned c = rowbase[r] + sk * (unsigned)L2;
-                    gcfloat *v4 = gy + 4 * (size_t)c;
-                        const float4 v = make_float4(v4[0], v4[1], v4[2], v4[3]);
-                    ky[r][0] = v.x;
-                    ky[r][1] = v.y;
-                    ky[r][2] = v.z;
-                    ky[r][3] = v.w;
-                }
+                const unsigned c = rowbase[r] + sk * l2;
+                gcfloat *v4 = gy + 4 * (size_t)c;
+                const float4 v = make_float4(v4[0], v4[1], v4[2], v4[3]);
+                ky[r][0] = v.x;
+                ky[r][1] = v.y;
+                ky[r][2] = v.z;
+                ky[r][3] = v.w;
             }
         };
         if (PF) load_coef();

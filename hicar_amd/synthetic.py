@@ -327,6 +327,157 @@ def config4(storage: int = _lib.STORE_BF16, nx=256, ny=64, nz=20, steps=200) -> 
     return case
 
 
+def dz_levels5(nz: int) -> np.ndarray:
+    """dz_k = 50 m * 1.07^k: 50 levels reach 20.3 km (the C1 stretching,
+    1.15^k, would put the 50th level at 384 km)."""
+    return 50.0 * 1.07 ** np.arange(nz, dtype=np.float64)
+
+
+def config5(nx=4096, ny=4096, nz=50, steps=20, storage: int = _lib.STORE_BF16,
+            workers: int = 16, log=None) -> Case:
+    """Large single domain, batched and mixed precision: dx = 250 m, fractal
+    terrain 0-4000 m with a 3000 m ridge line (Gaussian, 5 km half-width, at
+    a third of x), a 20 m/s westerly jet at 8 km (3 km wide) plus a
+    terrain-forced perturbation 0.2*U0*grad(h) of the terrain smoothed over
+    2 km, both in mass-flux form (u*rho*J is the level's mean rho*J times the
+    wind) with a perturbation profile that returns aloft what it carries
+    near the ground, w from continuity, Courant 0.5;
+    the 12 species of config 4, hydrometeor masses and numbers and the tracer
+    stored at 16 bits with per-species power-of-two scaling; order 2 with
+    FCT.  At 4096 x 4096 x 50 (839 M cells) the arrays are filled one level
+    at a time, levels in parallel threads, so that no float64 3-D temporary
+    exists; the formulas are those of the other configurations, level by
+    level (the random streams are keyed by level)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    say = log or (lambda m: None)
+    dom = Domain(nx, ny, nz, 250.0)
+    dz = dz_levels5(nz)
+    H = dz.sum()
+    zc = np.cumsum(dz) - 0.5 * dz
+    x = np.arange(nx) * dom.dx
+    ridge = 3000.0 * np.exp(-((x - nx * dom.dx / 3.0) / 5e3) ** 2)
+    terrain = np.maximum(fractal_terrain(nx, ny, 0.0, 4000.0, rng("C5", 0, "terrain")),
+                         ridge[None, :])
+    jac = (H - terrain) / H                                   # (ny, nx), every level
+    ju = np.empty((ny, nx + 1))
+    ju[:, 1:nx] = 0.5 * (jac[:, 1:] + jac[:, :-1])
+    ju[:, 0] = jac[:, 0]
+    ju[:, nx] = jac[:, nx - 1]
+    jv = np.empty((ny + 1, nx))
+    jv[1:ny] = 0.5 * (jac[1:] + jac[:-1])
+    jv[0] = jac[0]
+    jv[ny] = jac[ny - 1]
+    # the wind perturbation follows the terrain smoothed over 2 km
+    kx = np.fft.fftfreq(nx)[None, :]
+    ky = np.fft.fftfreq(ny)[:, None]
+    sig = 2000.0 / dom.dx
+    hs = np.real(np.fft.ifft2(np.fft.fft2(terrain) *
+                              np.exp(-2.0 * (np.pi * sig) ** 2 * (kx * kx + ky * ky))))
+    dhdx = (np.roll(hs, -1, axis=1) - np.roll(hs, 1, axis=1)) / (2 * dom.dx)
+    dhdy = (np.roll(hs, -1, axis=0) - np.roll(hs, 1, axis=0)) / (2 * dom.dx)
+    U0 = 20.0
+
+    def z_of(k):
+        return terrain + jac * zc[k]
+
+    # R_k: the level mean of rho*J; s_k: the perturbation's vertical profile,
+    # exp(-z/3 km) less a constant such that sum_k dz_k R_k s_k = 0, so the
+    # terrain-forced inflow below returns aloft and W closes at the top
+    R = np.array([float(np.mean(1.2 * np.exp(-z_of(k) / 8000.0) * jac)) for k in range(nz)])
+    e = np.exp(-zc / 3000.0)
+    prof = e - float(np.sum(dz * R * e) / np.sum(dz * R))
+
+    def level_fields(k):
+        """Metric, density and horizontal winds of level k (as _metric,
+        _fractal_winds with a jet profile); returns the level's horizontal
+        mass-flux divergence and the interface density for w."""
+        z = z_of(k)
+        dom.jacobian[:, k, :] = jac
+        dom.jacobian_u[:, k, :] = ju
+        dom.jacobian_v[:, k, :] = jv
+        dom.advection_dz[:, k, :] = dz[k]
+        if k < nz - 1:
+            dzi, dzu = dz[k] * jac, dz[k + 1] * jac
+            dom.jacobian_w[:, k, :] = (dzi * jac + dzu * jac) / (dzi + dzu)
+        else:
+            dom.jacobian_w[:, k, :] = jac
+        dom.density[:, k, :] = 1.2 * np.exp(-z / 8000.0)
+        rho = dom.density[:, k, :].astype(np.float64)  # the device's density
+        # mass-flux form: u*rho*J = R_k * (jet + 0.2*U0*dh/dx*s_k), the jet
+        # 20 m/s at 8 km (3 km wide) where rho*J equals its level mean
+        jet = U0 * np.exp(-((zc[k] - 8000.0) / 3000.0) ** 2)
+        ru = _face_x(rho) * dom.jacobian_u[:, k, :]
+        rv = _face_y(rho) * dom.jacobian_v[:, k, :]
+        u = R[k] * (jet + _face_x(0.2 * U0 * dhdx * prof[k])) / ru
+        v = R[k] * _face_y(0.2 * U0 * dhdy * prof[k]) / rv
+        dom.u[:, k, :] = u
+        dom.v[:, k, :] = v
+        Ur = u * _face_x(rho) * dom.jacobian_u[:, k, :] / dom.dx
+        Vr = v * _face_y(rho) * dom.jacobian_v[:, k, :] / dom.dx
+        divh = (Ur[:, 1:] - Ur[:, :-1]) + (Vr[1:] - Vr[:-1])
+        if k < nz - 1:
+            rz = 0.5 * (np.float32(1.2 * np.exp(-z_of(k + 1) / 8000.0)).astype(np.float64) + rho)
+        else:
+            rz = rho
+        umax = max(np.abs(u).max(), np.abs(v).max()) / dom.dx
+        return divh, rz, umax
+
+    say(f"C5 {nx}x{ny}x{nz}: metric, density and winds ...")
+    with ThreadPoolExecutor(workers) as ex:
+        lev = list(ex.map(level_fields, range(nz)))
+    # w from continuity, bottom up (as _diagnose_w)
+    Wr = np.zeros((ny, nx))
+    cmax = max(l[2] for l in lev)
+    jw = dom.jacobian_w
+    for k in range(nz):
+        Wr = Wr - dz[k] * lev[k][0]
+        w = Wr / (jw[:, k, :].astype(np.float64) * lev[k][1])
+        dom.w[:, k, :] = w
+        cmax = max(cmax, np.abs(w).max() / dz[k])
+    del lev
+    dt = float(np.float32(0.5 / cmax))
+    opts = Options()
+    opts.adv_options.mpdata_order = 2
+    opts.adv_options.MPDATA_FCT = True
+    opts.parameters.advect_density = True
+    opts.kernel_variant = 1
+    opts.vars_to_advect = list(C4_SPECIES)
+
+    def species_level(args):
+        name, k = args
+        r = rng("C5", k, name)
+        shape = (ny, nx)
+        z = z_of(k)
+        if name == "theta":
+            return 290.0 + 4e-3 * z + 0.2 * r.standard_normal(shape)
+        if name == "qv":
+            return 0.012 * np.exp(-z / 2500.0) * (1.0 + 0.05 * r.standard_normal(shape))
+        if name in NUMBERS:
+            return _lognormal(r, shape, 1e5)
+        q = _lognormal(r, shape, 1e-4)
+        q[rng("C5", k, name + "/mask").random(shape) < 0.6] = 0.0
+        return q
+
+    values = {}
+    for name in C4_SPECIES:
+        say(f"C5 species {name} ...")
+        a = np.empty(dom.shape, np.float32)
+
+        def fill(k, name=name, a=a):
+            a[:, k, :] = species_level((name, k))
+
+        with ThreadPoolExecutor(workers) as ex:
+            list(ex.map(fill, range(nz)))
+        values[name] = a
+    dom.set_var_lists(opts, values)
+    del values
+    for n, a in dom.species.items():
+        if n not in ("qv", "theta"):
+            dom.storage[n] = (storage, pow2_scale(a, storage))
+    return Case("C5", dom, opts, dt, steps, 1, 2, {"dz": "50*1.07^k"})
+
+
 def test_mpdata_case(nloops_steps: int = 392, fct: bool = True) -> Case:
     """The scenario of src/tests/test_mpdata.f90:51-111: nx=3, ny=100, nz=3,
     v = cfl = 0.25, u = w = 0, dz = rho = 1, mpdata_order 2, a step from 1

@@ -1,0 +1,53 @@
+"""The large single domain, batched and mixed precision: 4096 x 4096 x 50
+(839 M cells), the 12 species of the precision case with the hydrometeor
+masses and numbers and the tracer stored in BF16 (per-species power-of-two
+scaling), mpdata_order 2 with FCT, 20 steps at Courant 0.5.  About 180 GB
+of device memory.  Too large for the oracle; checked through properties that
+do not depend on its size:
+  - every value stays finite;
+  - with FCT no species goes negative (all start non-negative);
+  - a constant tracer stays constant: the synthetic w balances the scheme's
+    Courant numbers, so MPDATA moves nothing but rounding, and the BF16
+    value 2.5 is exact; its FP64 mass budget does not move either.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from hicar_amd import synthetic
+from hicar_amd.advection import MpdataAdvection
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 20
+
+
+def test_c5_full_size_mixed_precision_properties():
+    t0 = time.perf_counter()
+    case = synthetic.config5(steps=STEPS)
+    d = case.domain
+    assert d.nx * d.ny * d.nz == 838_860_800 and len(d.species) == 12
+    d.species["qtr"][...] = 2.5
+    kind = d.storage["qtr"][0]
+    d.storage["qtr"] = (kind, synthetic.pow2_scale(d.species["qtr"], kind))
+    assert d.storage["qtr"][1] == 2  # 2.5 = 0.625 * 2^2, exact in BF16
+    t1 = time.perf_counter()
+    st = MpdataAdvection(d, case.options)
+    try:
+        m0 = st.mass_budget()
+        st.run(case.dt, STEPS, case.bc_x, case.bc_y)
+        m1 = st.mass_budget()
+        st.download()
+    finally:
+        st.close()
+    t2 = time.perf_counter()
+    print(f"C5: generated in {t1 - t0:.0f} s, uploaded and {STEPS} steps in {t2 - t1:.0f} s")
+    names = list(d.species)
+    for n, a in d.species.items():
+        assert np.isfinite(a).all(), n
+        assert a.min() >= 0.0, (n, float(a.min()))
+    c = d.species["qtr"].astype(np.float64)
+    assert np.abs(c / 2.5 - 1).max() < 1e-5
+    k = names.index("qtr")
+    assert abs(m1[k] / m0[k] - 1) < 1e-6, (m0[k], m1[k])

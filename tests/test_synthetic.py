@@ -1,0 +1,69 @@
+"""The synthetic workloads' winds: w is diagnosed so that the scheme's own
+mass fluxes are discretely non-divergent (mpdata_compute_wind,
+adv_mpdata.f90:492-498: U = u*rho_face*J_u/dx, V likewise, W = w*J_w*rho_i),
+and the level-by-level generator of the 4096 x 4096 x 50 configuration keeps
+that, at small sizes."""
+import numpy as np
+import pytest
+
+from hicar_amd import _lib, synthetic
+from hicar_amd.synthetic import _face_x, _face_y
+
+
+def divergence(d):
+    """max |(U(i+1)-U(i)) + (V(j+1)-V(j)) + (W(k)-W(k-1))/dz_k| over the
+    max flux magnitude, in float64 from the float32 fields."""
+    rho = d.density.astype(np.float64)
+    U = d.u * _face_x(rho) * d.jacobian_u / d.dx
+    V = d.v * _face_y(rho) * d.jacobian_v / d.dx
+    rz = np.empty_like(rho)
+    rz[:, :-1] = 0.5 * (rho[:, 1:] + rho[:, :-1])
+    rz[:, -1] = rho[:, -1]
+    W = d.w * d.jacobian_w.astype(np.float64) * rz
+    dz = d.advection_dz.astype(np.float64)
+    Wb = np.concatenate([np.zeros_like(W[:, :1]), W[:, :-1]], axis=1)
+    div = (U[:, :, 1:] - U[:, :, :-1]) + (V[1:] - V[:-1]) + (W - Wb) / dz
+    scale = max(np.abs(U).max(), np.abs(V).max(), (np.abs(W) / dz).max())
+    return float(np.abs(div).max() / scale)
+
+
+@pytest.fixture(scope="module")
+def c5():
+    return synthetic.config5(nx=96, ny=64, nz=50, steps=2, workers=4)
+
+
+def test_c5_winds_are_non_divergent_for_the_scheme(c5):
+    d = c5.domain
+    assert divergence(d) < 1e-5
+    # the perturbation's return flow closes W at the model top, so w stays
+    # physical (a few m/s) up to the thin top levels
+    assert np.abs(d.w[:, -1]).max() < 1e-3
+    assert np.abs(d.w).max() < 10.0
+    assert 15.0 < d.u.max() < 30.0
+
+
+def test_c5_species_storage_and_courant(c5):
+    d = c5.domain
+    assert list(d.species) == list(synthetic.C4_SPECIES)
+    for n, a in d.species.items():
+        assert a.dtype == np.float32 and a.min() >= 0.0, n
+        if n in ("qv", "theta"):
+            assert n not in d.storage
+        else:
+            kind, e = d.storage[n]
+            assert kind == _lib.STORE_BF16 and 0.5 <= np.abs(a).max() / 2.0 ** e <= 1.0, n
+    dz = d.advection_dz.astype(np.float64)
+    c = c5.dt * max(np.abs(d.u).max() / d.dx, np.abs(d.v).max() / d.dx, (np.abs(d.w) / dz).max())
+    assert abs(c - 0.5) < 1e-5
+
+
+def test_c5_generation_is_reproducible(c5):
+    again = synthetic.config5(nx=96, ny=64, nz=50, steps=2, workers=2)
+    for n in c5.domain.FIELDS:
+        np.testing.assert_array_equal(getattr(c5.domain, n), getattr(again.domain, n), err_msg=n)
+    for n, a in c5.domain.species.items():
+        np.testing.assert_array_equal(a, again.domain.species[n], err_msg=n)
+
+
+def test_c2_winds_are_non_divergent_for_the_scheme():
+    assert divergence(synthetic.config2(nx=64, ny=48, nz=30).domain) < 1e-5
