@@ -23,6 +23,16 @@ __host__ __device__ inline size_t ix(const Dims &d, int i, int k, int j)
     return (size_t)i + (size_t)d.nx * ((size_t)k + (size_t)d.nz * (size_t)j);
 }
 
+// Blocks of a 1-D launch are dealt round-robin over the 8 XCDs, each with
+// its own L2.  This renumbers them so that each XCD takes a contiguous range
+// of the block order: blocks that read each other's rows then share an L2.
+__device__ __forceinline__ unsigned xcd_block()
+{
+    const unsigned L = blockIdx.x, B = gridDim.x;
+    const unsigned q = B / 8, rem = B % 8, x = L % 8, slot = L / 8;
+    return x < rem ? x * (q + 1) + slot : rem * (q + 1) + (x - rem) * q + slot;
+}
+
 // Kernel ids for the profiling accumulators (names in kernel_names()).
 enum KernelId {
     K_COURANT = 0,

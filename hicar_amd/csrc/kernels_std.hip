@@ -187,15 +187,6 @@ __device__ __forceinline__ float up2(float u, float qm, float qp, float dm, floa
 
 constexpr int CW = 63;  // columns a wave updates (lane 63 supplies a face only)
 
-// Blocks are dealt round-robin over the 8 XCDs; renumber them so that each
-// XCD takes a contiguous range of the block order.  The launch is 1-D.
-__device__ __forceinline__ unsigned xcd_block()
-{
-    const unsigned L = blockIdx.x, B = gridDim.x;
-    const unsigned q = B / 8, rem = B % 8, x = L % 8, slot = L / 8;
-    return x < rem ? x * (q + 1) + slot : rem * (q + 1) + (x - rem) * q + slot;
-}
-
 // q(k0-3 .. k0+3) of a column (zero outside it): w[0..5] is the stencil of
 // the face under level k0, w[1..6] the window the march starts with
 __device__ __forceinline__ void load_window7(float (&w)[7], G<float> q, off c0, int k0, int nz,
