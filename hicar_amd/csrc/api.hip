@@ -14,10 +14,14 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/icar_adv.h"
@@ -43,6 +47,81 @@ struct TimedLaunch {
     int id;
     hipEvent_t a, b;
     double bytes;
+};
+
+// A memcpy split over a few persistent threads and the caller: the CPU side
+// of the host transfers, where one thread copying pageable memory is what
+// limits the runtime's own staged copies.
+class CopyPool {
+public:
+    explicit CopyPool(int workers)
+    {
+        for (int i = 0; i < workers; i++) th_.emplace_back([this, i] { run(i + 1); });
+    }
+    ~CopyPool()
+    {
+        {
+            std::lock_guard<std::mutex> l(mu_);
+            quit_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    void copy(void *dst, const void *src, size_t n)
+    {
+        {
+            std::lock_guard<std::mutex> l(mu_);
+            dst_ = static_cast<char *>(dst);
+            src_ = static_cast<const char *>(src);
+            n_ = n;
+            left_ = (int)th_.size();
+            gen_++;
+        }
+        cv_.notify_all();
+        part(0);
+        std::unique_lock<std::mutex> l(mu_);
+        done_.wait(l, [this] { return left_ == 0; });
+    }
+
+private:
+    // part p of parts() contiguous pieces, cut at 64-byte boundaries
+    void part(int p) const
+    {
+        const size_t parts = th_.size() + 1;
+        const size_t a = p == 0 ? 0 : (n_ * p / parts) & ~(size_t)63;
+        const size_t b = p + 1 == (int)parts ? n_ : (n_ * (p + 1) / parts) & ~(size_t)63;
+        if (b > a) memcpy(dst_ + a, src_ + a, b - a);
+    }
+    void run(int p)
+    {
+        unsigned seen = 0;
+        std::unique_lock<std::mutex> l(mu_);
+        for (;;) {
+            cv_.wait(l, [&] { return quit_ || gen_ != seen; });
+            if (quit_) return;
+            seen = gen_;
+            l.unlock();
+            part(p);
+            l.lock();
+            if (--left_ == 0) done_.notify_one();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    char *dst_ = nullptr;
+    const char *src_ = nullptr;
+    size_t n_ = 0;
+    int left_ = 0;
+    unsigned gen_ = 0;
+    bool quit_ = false;
+};
+
+// one host<->device copy of a transfer
+struct XferItem {
+    char *host;
+    char *dev;
+    size_t bytes;
 };
 
 struct Handle {
@@ -96,6 +175,12 @@ struct Handle {
         size_t off, n;
     };
     std::vector<SnapItem> snap_items;
+    // upload / download through pinned staging: two chunks (the DMA of one
+    // overlaps the CPU copy of the other), their events and the copy pool
+    char *xfer_pin[2] = {};
+    hipEvent_t xfer_ev[2] = {};
+    std::unique_ptr<CopyPool> xfer_pool;
+    bool xfer_failed = false;  // pinned staging unavailable: the runtime's copies
     // profiling
     bool prof = false;
     std::vector<TimedLaunch> pending;
@@ -244,6 +329,11 @@ void free_all(Handle *h)
     if (h->snap_ready) (void)hipEventDestroy(h->snap_ready);
     if (h->snap_done) (void)hipEventDestroy(h->snap_done);
     if (h->copy_stream) (void)hipStreamDestroy(h->copy_stream);
+    for (int b = 0; b < 2; b++) {
+        if (h->xfer_pin[b]) (void)hipHostFree(h->xfer_pin[b]);
+        if (h->xfer_ev[b]) (void)hipEventDestroy(h->xfer_ev[b]);
+    }
+    h->xfer_pool.reset();
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     fast_batch_destroy(h->batch);
     fast_destroy(h->fast);
@@ -638,14 +728,115 @@ size_t strip_table(Handle *h, const Box &b, HaloStrips *hs, bool as_source)
     return bytes;
 }
 
+// Host transfers of at least XFER_MIN bytes go through the handle's pinned
+// staging in XFER_CHUNK pieces: the runtime stages pageable copies through
+// small buffers with one CPU thread; here the DMA of one chunk overlaps the
+// copy of the previous one by several threads.  Smaller transfers, or a
+// process where pinned memory cannot be had, use the runtime's copies.
+constexpr size_t XFER_CHUNK = 32u << 20, XFER_MIN = 8u << 20;
+constexpr int XFER_THREADS = 7;  // copy-pool workers besides the caller
+
+CopyPool &copy_pool(Handle *h)
+{
+    if (!h->xfer_pool) h->xfer_pool.reset(new CopyPool(XFER_THREADS));
+    return *h->xfer_pool;
+}
+
+bool xfer_ready(Handle *h)
+{
+    if (h->xfer_failed) return false;
+    if (h->xfer_pin[0]) return true;
+    for (int b = 0; b < 2; b++) {
+        // non-coherent: cacheable for the CPU side of the copies
+        if (hipHostMalloc((void **)&h->xfer_pin[b], XFER_CHUNK, hipHostMallocNonCoherent) !=
+                hipSuccess ||
+            hipEventCreateWithFlags(&h->xfer_ev[b], hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            h->xfer_failed = true;
+            return false;
+        }
+    }
+    copy_pool(h);
+    return true;
+}
+
+size_t xfer_bytes(const std::vector<XferItem> &items)
+{
+    size_t n = 0;
+    for (const auto &it : items) n += it.bytes;
+    return n;
+}
+
+// device -> host, in stream order after the work enqueued so far; returns
+// once the host arrays hold the data
+int xfer_down(Handle *h, const std::vector<XferItem> &items)
+{
+    if (xfer_bytes(items) < XFER_MIN || !xfer_ready(h)) {
+        for (const auto &it : items)
+            HIPCHK(h, hipMemcpyAsync(it.host, it.dev, it.bytes, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    struct Piece {
+        char *host;
+        const char *dev;
+        size_t n;
+    };
+    std::vector<Piece> ps;
+    for (const auto &it : items)
+        for (size_t o = 0; o < it.bytes; o += XFER_CHUNK)
+            ps.push_back({it.host + o, it.dev + o, std::min(XFER_CHUNK, it.bytes - o)});
+    for (size_t i = 0; i <= ps.size(); i++) {
+        if (i < ps.size()) {  // chunk i into buffer i%2 (its previous contents are copied out)
+            HIPCHK(h, hipMemcpyAsync(h->xfer_pin[i & 1], ps[i].dev, ps[i].n, hipMemcpyDeviceToHost,
+                                     h->stream));
+            HIPCHK(h, hipEventRecord(h->xfer_ev[i & 1], h->stream));
+        }
+        if (i > 0) {  // chunk i-1 out of its buffer while chunk i moves
+            const size_t b = (i - 1) & 1;
+            HIPCHK(h, hipEventSynchronize(h->xfer_ev[b]));
+            copy_pool(h).copy(ps[i - 1].host, h->xfer_pin[b], ps[i - 1].n);
+        }
+    }
+    return 0;
+}
+
+// host -> device, enqueued on the handle's stream; the host arrays may be
+// changed once this returns (their contents are in the staging or on the
+// device), the device copies are complete in stream order
+int xfer_up(Handle *h, const std::vector<XferItem> &items)
+{
+    if (xfer_bytes(items) < XFER_MIN || !xfer_ready(h)) {
+        for (const auto &it : items)
+            HIPCHK(h, hipMemcpyAsync(it.dev, it.host, it.bytes, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    size_t i = 0;
+    for (const auto &it : items)
+        for (size_t o = 0; o < it.bytes; o += XFER_CHUNK, i++) {
+            const size_t b = i & 1, n = std::min(XFER_CHUNK, it.bytes - o);
+            if (i >= 2) HIPCHK(h, hipEventSynchronize(h->xfer_ev[b]));  // chunk i-2 has left
+            copy_pool(h).copy(h->xfer_pin[b], it.host + o, n);
+            HIPCHK(h, hipMemcpyAsync(it.dev + o, h->xfer_pin[b], n, hipMemcpyHostToDevice, h->stream));
+            HIPCHK(h, hipEventRecord(h->xfer_ev[b], h->stream));
+        }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
 // Completes a pending snapshot into the host arrays.
 int finish_snapshot(Handle *h)
 {
     if (!h->snap_pending) return 0;
     h->snap_pending = false;
     HIPCHK(h, hipEventSynchronize(h->snap_done));
-    for (const auto &it : h->snap_items)
-        memcpy(it.host, h->snap_host + it.off, it.n * sizeof(float));
+    for (const auto &it : h->snap_items) {
+        if (it.n * sizeof(float) >= XFER_MIN)
+            copy_pool(h).copy(it.host, h->snap_host + it.off, it.n * sizeof(float));
+        else
+            memcpy(it.host, h->snap_host + it.off, it.n * sizeof(float));
+    }
     h->snap_items.clear();
     return 0;
 }
@@ -858,24 +1049,31 @@ int icar_adv_upload(void *handle, unsigned long long mask)
     Handle *h;
     int rc = check_handle(handle, &h);
     if (rc) return rc;
+    // FP32 arrays in one pipelined transfer; each 16-bit species through the
+    // FP32 staging and the pack kernel (stream order keeps the staging's uses
+    // apart)
+    std::vector<XferItem> items;
     for (int f = 0; f < ICAR_ADV_NFIELDS; f++) {
         if (!(mask & ICAR_ADV_MASK_FIELD(f)) || !h->host_field[f]) continue;
-        HIPCHK(h, hipMemcpyAsync(h->dev_field[f], h->host_field[f],
-                                 field_elems(h, f) * sizeof(float), hipMemcpyHostToDevice,
-                                 h->stream));
+        items.push_back({(char *)h->host_field[f], (char *)h->dev_field[f],
+                         field_elems(h, f) * sizeof(float)});
     }
     for (int s = 0; s < h->nspecies; s++) {
         SpeciesSlot &sl = h->sp[s];
         if (!(mask & ICAR_ADV_MASK_SPECIES(s)) || !sl.bound) continue;
-        if (sl.storage == ICAR_ADV_STORE_F32) {
-            HIPCHK(h, hipMemcpyAsync(species_dev(h, s), sl.host, h->N * sizeof(float), hipMemcpyHostToDevice,
-                                     h->stream));
-        } else {
-            HIPCHK(h, hipMemcpyAsync(h->stage, sl.host, h->N * sizeof(float),
-                                     hipMemcpyHostToDevice, h->stream));
-            timed(h, K_CONVERT, 6.0 * h->N,
-                  [&] { launch_pack(h->stream, h->N, h->stage, species_dev(h, s), sl.storage, sl.pow2); });
-        }
+        if (sl.storage == ICAR_ADV_STORE_F32)
+            items.push_back({(char *)sl.host, (char *)species_dev(h, s), h->N * sizeof(float)});
+    }
+    rc = xfer_up(h, items);
+    if (rc) return rc;
+    for (int s = 0; s < h->nspecies; s++) {
+        SpeciesSlot &sl = h->sp[s];
+        if (!(mask & ICAR_ADV_MASK_SPECIES(s)) || !sl.bound || sl.storage == ICAR_ADV_STORE_F32)
+            continue;
+        rc = xfer_up(h, {{(char *)sl.host, (char *)h->stage, h->N * sizeof(float)}});
+        if (rc) return rc;
+        timed(h, K_CONVERT, 6.0 * h->N,
+              [&] { launch_pack(h->stream, h->N, h->stage, species_dev(h, s), sl.storage, sl.pow2); });
     }
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -887,25 +1085,29 @@ int icar_adv_download(void *handle, unsigned long long mask)
     Handle *h;
     int rc = check_handle(handle, &h);
     if (rc) return rc;
+    std::vector<XferItem> items;
     for (int f = 0; f < ICAR_ADV_NFIELDS; f++) {
         if (!(mask & ICAR_ADV_MASK_FIELD(f)) || !h->host_field[f]) continue;
-        HIPCHK(h, hipMemcpyAsync(h->host_field[f], h->dev_field[f],
-                                 field_elems(h, f) * sizeof(float), hipMemcpyDeviceToHost,
-                                 h->stream));
+        items.push_back({(char *)h->host_field[f], (char *)h->dev_field[f],
+                         field_elems(h, f) * sizeof(float)});
     }
     for (int s = 0; s < h->nspecies; s++) {
         SpeciesSlot &sl = h->sp[s];
         if (!(mask & ICAR_ADV_MASK_SPECIES(s)) || !sl.bound) continue;
-        if (sl.storage == ICAR_ADV_STORE_F32) {
-            HIPCHK(h, hipMemcpyAsync(sl.host, species_dev(h, s), h->N * sizeof(float), hipMemcpyDeviceToHost,
-                                     h->stream));
-        } else {
-            timed(h, K_CONVERT, 6.0 * h->N,
-                  [&] { launch_unpack(h->stream, h->N, species_dev(h, s), h->stage, sl.storage, sl.pow2); });
-            HIPCHK(h, hipMemcpyAsync(sl.host, h->stage, h->N * sizeof(float),
-                                     hipMemcpyDeviceToHost, h->stream));
-            HIPCHK(h, hipStreamSynchronize(h->stream));
-        }
+        if (sl.storage == ICAR_ADV_STORE_F32)
+            items.push_back({(char *)sl.host, (char *)species_dev(h, s), h->N * sizeof(float)});
+    }
+    rc = xfer_down(h, items);
+    if (rc) return rc;
+    for (int s = 0; s < h->nspecies; s++) {
+        SpeciesSlot &sl = h->sp[s];
+        if (!(mask & ICAR_ADV_MASK_SPECIES(s)) || !sl.bound || sl.storage == ICAR_ADV_STORE_F32)
+            continue;
+        timed(h, K_CONVERT, 6.0 * h->N,
+              [&] { launch_unpack(h->stream, h->N, species_dev(h, s), h->stage, sl.storage, sl.pow2); });
+        HIPCHK(h, hipGetLastError());
+        rc = xfer_down(h, {{(char *)sl.host, (char *)h->stage, h->N * sizeof(float)}});
+        if (rc) return rc;
     }
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipStreamSynchronize(h->stream));
