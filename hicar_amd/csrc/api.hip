@@ -56,17 +56,15 @@ class CopyPool {
 public:
     explicit CopyPool(int workers)
     {
-        for (int i = 0; i < workers; i++) th_.emplace_back([this, i] { run(i + 1); });
-    }
-    ~CopyPool()
-    {
-        {
-            std::lock_guard<std::mutex> l(mu_);
-            quit_ = true;
+        th_.reserve(workers);
+        try {
+            for (int i = 0; i < workers; i++) th_.emplace_back([this, i] { run(i + 1); });
+        } catch (...) {  // the threads started so far end before the error leaves
+            stop();
+            throw;
         }
-        cv_.notify_all();
-        for (auto &t : th_) t.join();
     }
+    ~CopyPool() { stop(); }
     void copy(void *dst, const void *src, size_t n)
     {
         {
@@ -84,7 +82,17 @@ public:
     }
 
 private:
-    // part p of parts() contiguous pieces, cut at 64-byte boundaries
+    void stop()
+    {
+        {
+            std::lock_guard<std::mutex> l(mu_);
+            quit_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+        th_.clear();
+    }
+    // part p of the workers + 1 contiguous pieces, cut at 64-byte boundaries
     void part(int p) const
     {
         const size_t parts = th_.size() + 1;
@@ -736,10 +744,28 @@ size_t strip_table(Handle *h, const Box &b, HaloStrips *hs, bool as_source)
 constexpr size_t XFER_CHUNK = 32u << 20, XFER_MIN = 8u << 20;
 constexpr int XFER_THREADS = 7;  // copy-pool workers besides the caller
 
-CopyPool &copy_pool(Handle *h)
+// The copy pool, created on first use; nullptr when its threads cannot be
+// created (no exception leaves the C-ABI: the copies then run on the
+// calling thread).
+CopyPool *copy_pool(Handle *h)
 {
-    if (!h->xfer_pool) h->xfer_pool.reset(new CopyPool(XFER_THREADS));
-    return *h->xfer_pool;
+    if (!h->xfer_pool && !h->xfer_failed) {
+        try {
+            h->xfer_pool.reset(new CopyPool(XFER_THREADS));
+        } catch (...) {
+            h->xfer_failed = true;
+        }
+    }
+    return h->xfer_pool.get();
+}
+
+void host_copy(Handle *h, void *dst, const void *src, size_t n)
+{
+    CopyPool *pool = copy_pool(h);
+    if (pool)
+        pool->copy(dst, src, n);
+    else
+        memcpy(dst, src, n);
 }
 
 bool xfer_ready(Handle *h)
@@ -756,8 +782,7 @@ bool xfer_ready(Handle *h)
             return false;
         }
     }
-    copy_pool(h);
-    return true;
+    return copy_pool(h) != nullptr;
 }
 
 size_t xfer_bytes(const std::vector<XferItem> &items)
@@ -795,7 +820,7 @@ int xfer_down(Handle *h, const std::vector<XferItem> &items)
         if (i > 0) {  // chunk i-1 out of its buffer while chunk i moves
             const size_t b = (i - 1) & 1;
             HIPCHK(h, hipEventSynchronize(h->xfer_ev[b]));
-            copy_pool(h).copy(ps[i - 1].host, h->xfer_pin[b], ps[i - 1].n);
+            host_copy(h, ps[i - 1].host, h->xfer_pin[b], ps[i - 1].n);
         }
     }
     return 0;
@@ -817,7 +842,7 @@ int xfer_up(Handle *h, const std::vector<XferItem> &items)
         for (size_t o = 0; o < it.bytes; o += XFER_CHUNK, i++) {
             const size_t b = i & 1, n = std::min(XFER_CHUNK, it.bytes - o);
             if (i >= 2) HIPCHK(h, hipEventSynchronize(h->xfer_ev[b]));  // chunk i-2 has left
-            copy_pool(h).copy(h->xfer_pin[b], it.host + o, n);
+            host_copy(h, h->xfer_pin[b], it.host + o, n);
             HIPCHK(h, hipMemcpyAsync(it.dev + o, h->xfer_pin[b], n, hipMemcpyHostToDevice, h->stream));
             HIPCHK(h, hipEventRecord(h->xfer_ev[b], h->stream));
         }
@@ -833,7 +858,7 @@ int finish_snapshot(Handle *h)
     HIPCHK(h, hipEventSynchronize(h->snap_done));
     for (const auto &it : h->snap_items) {
         if (it.n * sizeof(float) >= XFER_MIN)
-            copy_pool(h).copy(it.host, h->snap_host + it.off, it.n * sizeof(float));
+            host_copy(h, it.host, h->snap_host + it.off, it.n * sizeof(float));
         else
             memcpy(it.host, h->snap_host + it.off, it.n * sizeof(float));
     }
