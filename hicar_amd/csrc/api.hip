@@ -17,6 +17,7 @@
 #include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -742,16 +743,27 @@ size_t strip_table(Handle *h, const Box &b, HaloStrips *hs, bool as_source)
 // copy of the previous one by several threads.  Smaller transfers, or a
 // process where pinned memory cannot be had, use the runtime's copies.
 constexpr size_t XFER_CHUNK = 32u << 20, XFER_MIN = 8u << 20;
-constexpr int XFER_THREADS = 7;  // copy-pool workers besides the caller
+
+// copy-pool workers besides the caller: ICAR_ADV_COPY_THREADS, default 7
+// (0: every copy on the calling thread)
+int xfer_threads()
+{
+    static const int n = [] {
+        const char *e = std::getenv("ICAR_ADV_COPY_THREADS");
+        const int v = e ? std::atoi(e) : 7;
+        return std::max(0, std::min(v, 64));
+    }();
+    return n;
+}
 
 // The copy pool, created on first use; nullptr when its threads cannot be
 // created (no exception leaves the C-ABI: the copies then run on the
 // calling thread).
 CopyPool *copy_pool(Handle *h)
 {
-    if (!h->xfer_pool && !h->xfer_failed) {
+    if (!h->xfer_pool && !h->xfer_failed && xfer_threads() > 0) {
         try {
-            h->xfer_pool.reset(new CopyPool(XFER_THREADS));
+            h->xfer_pool.reset(new CopyPool(xfer_threads()));
         } catch (...) {
             h->xfer_failed = true;
         }
@@ -782,7 +794,8 @@ bool xfer_ready(Handle *h)
             return false;
         }
     }
-    return copy_pool(h) != nullptr;
+    copy_pool(h);
+    return !h->xfer_failed;
 }
 
 size_t xfer_bytes(const std::vector<XferItem> &items)
