@@ -901,12 +901,12 @@ int std_check(const icar_adv_grid *g, const icar_adv_opts *o)
                        o->h_order, o->flux_corr == 1 ? " with the limiter" : "", need, hw, hs);
     if (o->v_order == 5 && g->kme - g->kms + 1 < 4)
         return set_err(nullptr, ICAR_ADV_ESHAPE, "v_order 5 needs at least 4 levels");
-    // the kernels form byte offsets of the (scale_in, scale_out) pairs in 32 bits
+    // the kernels form the byte offsets of float arrays in 32 bits
     const long long cells = (long long)(g->ime - g->ims + 2) * (g->jme - g->jms + 2) *
                             (g->kme - g->kms + 1);
-    if (cells >= (1LL << 29))
+    if (cells >= (1LL << 30))
         return set_err(nullptr, ICAR_ADV_ERANGE,
-                       "the standard scheme takes at most 2^29 cells per domain (%lld)", cells);
+                       "the standard scheme takes at most 2^30 cells per domain (%lld)", cells);
     return 0;
 }
 

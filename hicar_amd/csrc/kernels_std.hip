@@ -29,9 +29,9 @@ namespace icaradv {
 
 namespace {
 
-// Element offsets are 32-bit and byte offsets are formed in 32 bits (the
-// host limits a domain to 2^29 cells), so that a load is a uniform base in
-// scalar registers plus one 32-bit vector offset.
+// Element offsets are 32-bit and the byte offsets of float arrays are
+// formed in 32 bits (the host limits a domain to 2^30 cells), so that a load
+// is a uniform base in scalar registers plus one 32-bit vector offset.
 typedef uint32_t off;
 typedef ptrdiff_t soff;  // signed offsets where loads go through 64-bit pointers
 
@@ -46,14 +46,16 @@ struct G {
     }
 };
 
-// (scale_in, scale_out) pairs as two adjacent floats
+// (scale_in, scale_out) pairs: one 64-bit load, its byte offset formed in
+// 64 bits (8-byte elements of up to 2^30 cells)
 template <>
 struct G<float2> {
     const float2 *p;
     __device__ __forceinline__ float2 operator[](off i) const
     {
-        const G<float> f{reinterpret_cast<const float *>(p)};
-        return make_float2(f[2 * i], f[2 * i + 1]);
+        typedef __attribute__((address_space(1))) const uint64_t gcu64;
+        const uint64_t v = ((gcu64 *)p)[i];
+        return make_float2(__uint_as_float((uint32_t)v), __uint_as_float((uint32_t)(v >> 32)));
     }
 };
 
@@ -66,9 +68,8 @@ __device__ __forceinline__ void put(float *p, off i, float v)
 
 __device__ __forceinline__ void put(float2 *p, off i, float2 v)
 {
-    float *f = reinterpret_cast<float *>(p);
-    put(f, 2 * i, v.x);
-    put(f, 2 * i + 1, v.y);
+    typedef __attribute__((address_space(1))) uint64_t gu64;
+    ((gu64 *)p)[i] = (uint64_t)__float_as_uint(v.x) | ((uint64_t)__float_as_uint(v.y) << 32);
 }
 
 // A block of cells [i_lo, i_hi] x [j_lo, j_hi] (0-based), all levels, cut

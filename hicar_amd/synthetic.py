@@ -18,7 +18,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from .domain import Domain, Options, kADV_STD, halo_size
+from .domain import Domain, Options, kADV_STD, kFLUXCOR_MONO, halo_size
 
 HYDRO_MASS = ("qc", "qr", "qs", "qi", "qg")
 NUMBERS = ("ni", "nr", "ns", "ng")
@@ -334,7 +334,7 @@ def dz_levels5(nz: int) -> np.ndarray:
 
 
 def config5(nx=4096, ny=4096, nz=50, steps=20, storage: int = _lib.STORE_BF16,
-            workers: int = 16, log=None) -> Case:
+            workers: int = 16, log=None, scheme: str = "mpdata", names=None) -> Case:
     """Large single domain, batched and mixed precision: dx = 250 m, fractal
     terrain 0-4000 m with a 3000 m ridge line (Gaussian, 5 km half-width, at
     a third of x), a 20 m/s westerly jet at 8 km (3 km wide) plus a
@@ -347,11 +347,29 @@ def config5(nx=4096, ny=4096, nz=50, steps=20, storage: int = _lib.STORE_BF16,
     FCT.  At 4096 x 4096 x 50 (839 M cells) the arrays are filled one level
     at a time, levels in parallel threads, so that no float64 3-D temporary
     exists; the formulas are those of the other configurations, level by
-    level (the random streams are keyed by level)."""
+    level (the random streams are keyed by level).  With scheme "std" the
+    same domain is set up for the standard scheme (h_order 5, v_order 3, the
+    WRF limiter, RK3; all species FP32, `names` of them), w balancing its
+    mass fluxes with their dz-weighted interface density."""
     from concurrent.futures import ThreadPoolExecutor
 
     say = log or (lambda m: None)
-    dom = Domain(nx, ny, nz, 250.0)
+    std = scheme == "std"
+    opts = Options()
+    opts.parameters.advect_density = True
+    if std:
+        opts.physics.advection = kADV_STD
+        opts.adv_options.h_order = 5
+        opts.adv_options.v_order = 3
+        opts.adv_options.flux_corr = kFLUXCOR_MONO
+        opts.time_options.RK3 = True
+    else:
+        opts.adv_options.mpdata_order = 2
+        opts.adv_options.MPDATA_FCT = True
+        opts.kernel_variant = 1
+    names = list(names or C4_SPECIES)
+    opts.vars_to_advect = names
+    dom = Domain(nx, ny, nz, 250.0, halo=halo_size(opts))
     dz = dz_levels5(nz)
     H = dz.sum()
     zc = np.cumsum(dz) - 0.5 * dz
@@ -417,7 +435,12 @@ def config5(nx=4096, ny=4096, nz=50, steps=20, storage: int = _lib.STORE_BF16,
         Vr = v * _face_y(rho) * dom.jacobian_v[:, k, :] / dom.dx
         divh = (Ur[:, 1:] - Ur[:, :-1]) + (Vr[1:] - Vr[:-1])
         if k < nz - 1:
-            rz = 0.5 * (np.float32(1.2 * np.exp(-z_of(k + 1) / 8000.0)).astype(np.float64) + rho)
+            ra = np.float32(1.2 * np.exp(-z_of(k + 1) / 8000.0)).astype(np.float64)
+            if std:  # adv_std_compute_wind's interface density (advect.f90:475-476)
+                dk, da = np.float64(np.float32(dz[k])), np.float64(np.float32(dz[k + 1]))
+                rz = (rho * dk + ra * da) / (dk + da)
+            else:
+                rz = 0.5 * (ra + rho)
         else:
             rz = rho
         umax = max(np.abs(u).max(), np.abs(v).max()) / dom.dx
@@ -437,12 +460,6 @@ def config5(nx=4096, ny=4096, nz=50, steps=20, storage: int = _lib.STORE_BF16,
         cmax = max(cmax, np.abs(w).max() / dz[k])
     del lev
     dt = float(np.float32(0.5 / cmax))
-    opts = Options()
-    opts.adv_options.mpdata_order = 2
-    opts.adv_options.MPDATA_FCT = True
-    opts.parameters.advect_density = True
-    opts.kernel_variant = 1
-    opts.vars_to_advect = list(C4_SPECIES)
 
     def species_level(args):
         name, k = args
@@ -460,7 +477,7 @@ def config5(nx=4096, ny=4096, nz=50, steps=20, storage: int = _lib.STORE_BF16,
         return q
 
     values = {}
-    for name in C4_SPECIES:
+    for name in names:
         say(f"C5 species {name} ...")
         a = np.empty(dom.shape, np.float32)
 
@@ -472,10 +489,11 @@ def config5(nx=4096, ny=4096, nz=50, steps=20, storage: int = _lib.STORE_BF16,
         values[name] = a
     dom.set_var_lists(opts, values)
     del values
-    for n, a in dom.species.items():
-        if n not in ("qv", "theta"):
-            dom.storage[n] = (storage, pow2_scale(a, storage))
-    return Case("C5", dom, opts, dt, steps, 1, 2, {"dz": "50*1.07^k"})
+    if not std:
+        for n, a in dom.species.items():
+            if n not in ("qv", "theta"):
+                dom.storage[n] = (storage, pow2_scale(a, storage))
+    return Case("C5", dom, opts, dt, steps, 1, 2, {"dz": "50*1.07^k", "scheme": scheme})
 
 
 def test_mpdata_case(nloops_steps: int = 392, fct: bool = True) -> Case:

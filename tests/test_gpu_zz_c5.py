@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from hicar_amd import synthetic
-from hicar_amd.advection import MpdataAdvection
+from hicar_amd.advection import Advection, MpdataAdvection
 
 pytestmark = pytest.mark.gpu
 
@@ -51,3 +51,27 @@ def test_c5_full_size_mixed_precision_properties():
     assert np.abs(c / 2.5 - 1).max() < 1e-5
     k = names.index("qtr")
     assert abs(m1[k] / m0[k] - 1) < 1e-6, (m0[k], m1[k])
+
+
+def test_standard_scheme_past_2_29_cells():
+    """h_order 5, v_order 3, the WRF limiter and RK3, two steps: finite; a
+    species that starts non-negative stays so below the 2s-1 = 3 levels the
+    unlimited model-top flux reaches (up to FP32 rounding of nearly equal
+    fluxes); a constant tracer stays constant (w balances this scheme's
+    mass fluxes)."""
+    case = synthetic.config5(nx=4096, ny=4096, nz=33, steps=2, scheme="std", names=("qc", "qtr"))
+    d = case.domain
+    assert (d.nx + 2) * (d.ny + 2) * d.nz > 2 ** 29
+    d.species["qtr"][...] = 2.5
+    st = Advection(d, case.options)
+    try:
+        st.advect(case.dt)
+        st.advect(case.dt)
+        st.download()
+    finally:
+        st.close()
+    q = d.species["qc"]
+    assert np.isfinite(q).all()
+    assert q[:, : d.nz - 3].min() >= -1e-7 * q.max()
+    c = d.species["qtr"].astype(np.float64)
+    assert np.abs(c / 2.5 - 1).max() < 1e-5
