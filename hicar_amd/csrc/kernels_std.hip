@@ -400,7 +400,7 @@ __device__ __forceinline__ THalo tile_halo(const TLane &l, G<float> qin, G<float
 
 template <bool Q, int NP>
 __device__ __forceinline__ void stage_halo(const TLane &l, const THalo &h, float *sq, float *so,
-                                           float *sd, float2 *ss)
+                                           float *sd, float *ssi, float *sso)
 {
     if (Q) {
 #pragma unroll
@@ -410,7 +410,10 @@ __device__ __forceinline__ void stage_halo(const TLane &l, const THalo &h, float
     if (NP >= 1 && l.hod >= 0) {
         so[l.hod] = h.o;
         if (NP >= 2) sd[l.hod] = h.dq;
-        if (NP >= 3) ss[l.hod] = h.sc;
+        if (NP >= 3) {
+            ssi[l.hod] = h.sc.x;
+            sso[l.hod] = h.sc.y;
+        }
     }
 }
 
@@ -486,7 +489,7 @@ __global__ void __launch_bounds__(64 * (TY + 1))
         const bool top = k == d.nz - 1;
         if (FL) sq[l.pq] = zw[2];
         if (DQ) so[l.pp] = o0;
-        stage_halo<FL, DQ ? 1 : 0>(l, h, sq, so, nullptr, nullptr);
+        stage_halo<FL, DQ ? 1 : 0>(l, h, sq, so, nullptr, nullptr, nullptr);
         const float on = k + 2 < d.nz ? qo[c + 2 * sz] : 0.0f;
         TWind fn = f;
         if (k + 1 < l.k1) {
@@ -579,7 +582,7 @@ __global__ void __launch_bounds__(64 * (TY + 1)) k_std_scale(StdGeom g, StdBatch
         sq[l.pq] = zw[2];
         so[l.pp] = o0;
         sd[l.pp] = d0;
-        stage_halo<true, 2>(l, h, sq, so, sd, nullptr);
+        stage_halo<true, 2>(l, h, sq, so, sd, nullptr, nullptr);
         float on = 0.0f, dn = 0.0f;
         if (k + 2 < d.nz) {
             on = qo[c + 2 * sz];
@@ -680,7 +683,7 @@ __global__ void __launch_bounds__(64 * (TY + 1)) k_std_final(StdGeom g, StdBatch
     using T = Tile<H, TY>;
     __shared__ float sq[T::QR * T::QC];
     __shared__ float so[T::PR * T::PC], sd[T::PR * T::PC];
-    __shared__ float2 ss[T::PR * T::PC];
+    __shared__ float ssi[T::PR * T::PC], sso[T::PR * T::PC];  // scale_in, scale_out
     __shared__ float sy[(TY + 1) * 64];
     const Dims d = g.d;
     const TLane l = tile_lane<H, TY>(d, r);
@@ -717,8 +720,9 @@ __global__ void __launch_bounds__(64 * (TY + 1)) k_std_final(StdGeom g, StdBatch
         sq[l.pq] = zw[2];
         so[l.pp] = o0;
         sd[l.pp] = d0;
-        ss[l.pp] = s0;
-        stage_halo<true, 3>(l, h, sq, so, sd, ss);
+        ssi[l.pp] = s0.x;
+        sso[l.pp] = s0.y;
+        stage_halo<true, 3>(l, h, sq, so, sd, ssi, sso);
         float on = 0.0f, dn = 0.0f;
         float2 sn = make_float2(0.0f, 0.0f);
         if (k + 2 < d.nz) {
@@ -737,7 +741,8 @@ __global__ void __launch_bounds__(64 * (TY + 1)) k_std_final(StdGeom g, StdBatch
         float Fxw = 0.0f, Fzt = 0.0f;
         if (row) {
             const float uxw = up2(f.uw, so[l.pp - 1], o0, sd[l.pp - 1], d0);
-            Fxw = limit_face(face_h<H>(f.uw, sq, l.pq, 1) - uxw, s0, ss[l.pp - 1]) + uxw;
+            Fxw = limit_face(face_h<H>(f.uw, sq, l.pq, 1) - uxw, s0,
+                             make_float2(ssi[l.pp - 1], sso[l.pp - 1])) + uxw;
             if (top) {  // merged but not limited (:139-143 stop at kme)
                 float uzt = 0.5f * o0 * f.wt;
                 uzt = uzt + 0.5f * d0 * f.wt;
@@ -751,7 +756,8 @@ __global__ void __launch_bounds__(64 * (TY + 1)) k_std_final(StdGeom g, StdBatch
         // y: the south face, the north one is the next row's
         const int ps = l.pp - T::PC;
         const float uys = up2(f.vs, so[ps], o0, sd[ps], d0);
-        const float Fys = limit_face(face_h<H>(f.vs, sq, l.pq, T::QC) - uys, s0, ss[ps]) + uys;
+        const float Fys =
+            limit_face(face_h<H>(f.vs, sq, l.pq, T::QC) - uys, s0, make_float2(ssi[ps], sso[ps])) + uys;
         sy[l.w * 64 + l.L] = Fys;
         __syncthreads();
         if (l.cell)
