@@ -2,7 +2,7 @@
 
 The "exact" kernels (kernel_variant 0) keep the reference's IEEE FP32
 operation order, so they are compared element by element with the FP32
-oracle (bit-identical values); tolerance of the north star for longer runs:
+oracle (bit-identical values); the parity bar for longer runs is
 max|q - q_ref| / max|q_ref| <= 1e-5 per field.
 """
 import numpy as np
