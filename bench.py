@@ -368,6 +368,10 @@ def main() -> int:
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 48-domain ensemble")
     ap.add_argument("--no-std", action="store_true",
                     help="skip the standard scheme (RK3 + WRF limiter) measurement")
+    ap.add_argument("--no-handoff", action="store_true",
+                    help="skip the host transfer and snapshot measurement")
+    ap.add_argument("--no-wind", action="store_true",
+                    help="skip the wind-balance, dt and forcing kernels' timing")
     ap.add_argument("--workload", choices=("c2", "c5"), default="c2",
                     help="c2: the 1024x1024x30 single domain (the headline line); c5: the "
                          "4096x4096x50 mixed-precision domain alone")
@@ -402,7 +406,7 @@ def main() -> int:
         log(f"  {name}: {results[name]['value'] / 1e9:.2f} G cell-species/s, "
             f"{1e3 * t / args.steps:.2f} ms/step")
     wind = None
-    if args.variant == "all":
+    if args.variant == "all" and not args.no_wind:
         wind = measure_wind_steps(case)
         log("  wind steps: " + ", ".join(f"{k} {v['avg_ms']:.3f} ms ({v['achieved_GBps']:.0f} GB/s)"
                                          for k, v in wind.items()))
@@ -424,7 +428,7 @@ def main() -> int:
         log(f"  ensemble of 48: eager {ensemble['eager']['value'] / 1e9:.2f}, graph "
             f"{ensemble['graph']['value'] / 1e9:.2f} G cell-species/s")
     handoff = None
-    if args.variant == "all":
+    if args.variant == "all" and not args.no_handoff:
         handoff = measure_host_handoff(case, min(args.steps, 10), args.warmup)
         log(f"  host hand-off: species upload {handoff['upload_ms']:.1f} ms "
             f"({handoff['upload_GBps']:.1f} GB/s), download {handoff['download_ms']:.1f} ms "
