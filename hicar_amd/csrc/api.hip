@@ -1202,8 +1202,14 @@ int icar_adv_max_courant(void *handle, double out[3])
     if (!out) return set_err(h, ICAR_ADV_EINVAL, "NULL output");
     if (!h->have_dt) return set_err(h, ICAR_ADV_EINVAL, "advect has not been called yet");
     const int nb = mass_partials_blocks(h->d);
-    timed(h, K_REDUCE, 3.0 * 4 * cells(h),
-          [&] { launch_courant_max(h->stream, h->d, h->sf.U, h->sf.V, h->sf.Wd, h->d_partial); });
+    timed(h, K_REDUCE, (h->is_std ? 4.0 : 3.0) * 4 * cells(h), [&] {
+        if (h->is_std)  // the standard scheme's U_m, V_m, W_m of its last step
+            launch_courant_max(h->stream, h->d, h->Um, h->d.nx + 1, h->Vm, h->Wm,
+                               h->dev_field[ICAR_ADV_FIELD_ADVECTION_DZ], h->d_partial);
+        else
+            launch_courant_max(h->stream, h->d, h->sf.U, h->d.nx, h->sf.V, h->sf.Wd, nullptr,
+                               h->d_partial);
+    });
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipMemcpyAsync(h->h_partial.data(), h->d_partial, sizeof(double) * 3 * nb,
                              hipMemcpyDeviceToHost, h->stream));

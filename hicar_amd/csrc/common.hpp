@@ -100,9 +100,10 @@ void launch_dt_partials(hipStream_t s, Dims d, int its, int ite, int jts, int jt
                         float *part);
 void launch_mass(hipStream_t s, Dims d, const float *q, const float *jaco, const float *density,
                  const float *dz, double *partial);
-// FP64 Courant maxima partials: 3 doubles per block.
-void launch_courant_max(hipStream_t s, Dims d, const float *U, const float *V, const float *Wd,
-                        double *partial);
+// FP64 Courant maxima partials: 3 doubles per block.  U has unx faces per
+// row (nx or nx + 1); W is divided by dz unless dz is null.
+void launch_courant_max(hipStream_t s, Dims d, const float *U, int unx, const float *V,
+                        const float *W, const float *dz, double *partial);
 void launch_copy(hipStream_t s, size_t n, const float *src, float *dst);
 // Divergence check (debug): counts cells with |div| > 1e-3; out[0]=count, out[1]=max.
 void launch_divergence(hipStream_t s, Dims d, const float *U, const float *V, const float *W,

@@ -417,19 +417,25 @@ __global__ void __launch_bounds__(RED_THREADS)
     }
 }
 
+// Maxima of |U| over both x faces, |V| over both y faces and |W/dz| of the
+// interior cells.  U holds unx faces per row (nx for the MPDATA Courant
+// numbers, cell c's west face at c; nx + 1 for the standard scheme's U_m);
+// dz is null when W is already divided by it.
 __global__ void __launch_bounds__(RED_THREADS)
-    k_courant_max(Dims d, const float *__restrict__ U, const float *__restrict__ V,
-                  const float *__restrict__ Wd, double *partial)
+    k_courant_max(Dims d, const float *__restrict__ U, int unx, const float *__restrict__ V,
+                  const float *__restrict__ W, const float *__restrict__ dz, double *partial)
 {
     const size_t total = (size_t)(d.nx - 2) * d.nz * (d.ny - 2);
     double mu = 0.0, mv = 0.0, mw = 0.0;
     const size_t sj = (size_t)d.nx * d.nz;
+    const size_t extra = (size_t)(unx - d.nx);
     for (size_t n = (size_t)blockIdx.x * RED_THREADS + threadIdx.x; n < total;
          n += (size_t)gridDim.x * RED_THREADS) {
         const size_t c = interior_cell(d, n);
-        mu = fmax(mu, fmax(fabs((double)U[c]), fabs((double)U[c + 1])));
+        const size_t fu = c + extra * (c / (size_t)d.nx);  // + one face per row before c
+        mu = fmax(mu, fmax(fabs((double)U[fu]), fabs((double)U[fu + 1])));
         mv = fmax(mv, fmax(fabs((double)V[c]), fabs((double)V[c + sj])));
-        mw = fmax(mw, fabs((double)Wd[c]));
+        mw = fmax(mw, fabs((double)(dz ? W[c] / dz[c] : W[c])));
     }
     __shared__ double sh[3][RED_THREADS / 64];
     mu = wave_max(mu);
@@ -556,11 +562,11 @@ void launch_mass(hipStream_t s, Dims d, const float *q, const float *jaco, const
                        dz, partial);
 }
 
-void launch_courant_max(hipStream_t s, Dims d, const float *U, const float *V, const float *Wd,
-                        double *partial)
+void launch_courant_max(hipStream_t s, Dims d, const float *U, int unx, const float *V,
+                        const float *W, const float *dz, double *partial)
 {
-    hipLaunchKernelGGL(k_courant_max, dim3(red_blocks(d)), dim3(RED_THREADS), 0, s, d, U, V, Wd,
-                       partial);
+    hipLaunchKernelGGL(k_courant_max, dim3(red_blocks(d)), dim3(RED_THREADS), 0, s, d, U, unx, V,
+                       W, dz, partial);
 }
 
 void launch_divergence(hipStream_t s, Dims d, const float *U, const float *V, const float *W,

@@ -177,7 +177,9 @@ int icar_adv_mass_budget(void *handle, double *per_species);
 
 /* FP64 maxima over the interior of |U_m|, |V_m| and |W_m/dz| for the dt of
  * the last advect call (the cfl_strictness=3 measure of compute_dt,
- * time_step.f90:98-127, expressed as Courant numbers).  Synchronises. */
+ * time_step.f90:98-127, expressed as Courant numbers): the mass-flux Courant
+ * numbers of the scheme in use, MPDATA's (adv_mpdata.f90:489-498) or the
+ * standard scheme's (advect.f90:444-492).  Synchronises. */
 int icar_adv_max_courant(void *handle, double out[3]);
 
 /* = balance_uvw(domain, options) (src/physics/wind.f90:88-164, update

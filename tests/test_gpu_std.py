@@ -241,3 +241,31 @@ def test_std_rejects_16bit_storage():
     case.domain.storage["qv"] = (_lib.STORE_BF16, 0)
     with pytest.raises(_lib.IcarAdvError):
         Advection(case.domain, case.options).close()
+
+
+@pytest.mark.parametrize("adv_den", [True, False])
+def test_std_max_courant_is_the_last_steps_mass_flux_maxima(oracle32, adv_den):
+    """icar_adv_max_courant on the standard scheme reduces the U_m, V_m, W_m
+    of adv_std_compute_wind (advect.f90:444-492) for the last dt: maxima of
+    |U_m| over the x faces and |V_m| over the y faces of the interior cells,
+    and of |W_m/dz| (FP32 quotient) over those cells, equal to the oracle's
+    fluxes reduced in numpy."""
+    case = std_case(5, 3, 1, True, nx=37, ny=29, nz=8)
+    d, o = case.domain, case.options
+    o.parameters.advect_density = adv_den
+    r = np.random.default_rng(5)
+    d.v[...] = r.uniform(-4, 4, d.v.shape)
+    d.w[...] = r.uniform(-0.5, 0.5, d.w.shape)
+    st = Advection(d, o)
+    try:
+        st.advect(0.5 * case.dt)
+        st.advect(case.dt)
+        got = st.max_courant()
+    finally:
+        st.close()
+    Um, Vm, Wm, _ = oracle32.std_compute_wind(d, case.dt, adv_den)
+    inner = (slice(1, -1), slice(None), slice(1, -1))
+    want = (np.abs(Um[1:-1, :, 1:-1]).max(), np.abs(Vm[1:-1, :, 1:-1]).max(),
+            np.abs(Wm / d.advection_dz.astype(np.float32))[inner].max())
+    np.testing.assert_array_equal(got, np.array(want, np.float64))
+    assert got[1] > 0 and got[2] > 0
