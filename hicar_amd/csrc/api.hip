@@ -951,9 +951,11 @@ int icar_adv_create(const icar_adv_grid *grid, const icar_adv_opts *opts, void *
         return set_err(nullptr, (int)e, "hipStreamCreate failed: %s", hipGetErrorString(e));
     }
     const size_t fb = h->N * sizeof(float);
+    // MPDATA's Courant numbers and products; the standard scheme keeps its own
+    // (U_m, V_m, W_m and the divisors below) and leaves these null
     float **bufs[] = {&h->sf.U, &h->sf.V, &h->sf.W, &h->sf.G, &h->sf.Gz, &h->sf.Wd};
     for (float **p : bufs)
-        if (dalloc(h, (void **)p, fb)) {
+        if (!h->is_std && dalloc(h, (void **)p, fb)) {
             std::string m = h->err;
             free_all(h);
             return set_err(nullptr, ICAR_ADV_ENOMEM, "%s", m.c_str());

@@ -269,3 +269,27 @@ def test_std_max_courant_is_the_last_steps_mass_flux_maxima(oracle32, adv_den):
             np.abs(Wm / d.advection_dz.astype(np.float32))[inner].max())
     np.testing.assert_array_equal(got, np.array(want, np.float64))
     assert got[1] > 0 and got[2] > 0
+
+
+@pytest.mark.parametrize("bc_x,bc_y,graph", [(1, 2, False), (2, 0, False), (1, 2, True)])
+def test_std_run_with_boundary_conditions_matches_oracle(oracle32, bc_x, bc_y, graph):
+    """icar_adv_run's ring conditions after each standard step act on the
+    buffer the step wrote (the species swap buffers every step): 3 steps,
+    eager and replayed from a graph, against the oracle's step + apply_bc."""
+    case = std_case(5, 3, 1, True, nx=36, ny=30, nz=7)
+    d = case.domain
+    cur = {n: a.copy() for n, a in d.species.items()}
+    for _ in range(3):
+        cur = oracle32.std_advect(d, case.options, case.dt, cur)
+        for n in cur:
+            oracle32.apply_bc(cur[n], bc_x, bc_y)
+    st = Advection(d, case.options)
+    try:
+        st.set_graph(graph)
+        st.run(case.dt, 1, bc_x, bc_y)
+        st.run(case.dt, 2, bc_x, bc_y)
+        st.download()
+    finally:
+        st.close()
+    for n in cur:
+        np.testing.assert_array_equal(d.species[n], cur[n], err_msg=n)
