@@ -157,6 +157,9 @@ struct Handle {
     std::string err;
     FastState *fast = nullptr;
     FastBatch *batch = nullptr;  // this domain as a batch of one (fused path)
+    // counts species bindings: each may reallocate device buffers and the
+    // fused path's tables, which recorded graphs and ensembles point at
+    unsigned bind_gen = 0;
     // graph capture of icar_adv_run
     bool use_graph = false;
     hipGraphExec_t gexec = nullptr;
@@ -854,7 +857,10 @@ int xfer_up(Handle *h, const std::vector<XferItem> &items)
     for (const auto &it : items)
         for (size_t o = 0; o < it.bytes; o += XFER_CHUNK, i++) {
             const size_t b = i & 1, n = std::min(XFER_CHUNK, it.bytes - o);
-            if (i >= 2) HIPCHK(h, hipEventSynchronize(h->xfer_ev[b]));  // chunk i-2 has left
+            // the buffer's last DMA has left: chunk i-2's, or for i < 2 one a
+            // previous call enqueued (complete unless that call failed part way;
+            // an event never recorded returns at once)
+            HIPCHK(h, hipEventSynchronize(h->xfer_ev[b]));
             host_copy(h, h->xfer_pin[b], it.host + o, n);
             HIPCHK(h, hipMemcpyAsync(it.dev + o, h->xfer_pin[b], n, hipMemcpyHostToDevice, h->stream));
             HIPCHK(h, hipEventRecord(h->xfer_ev[b], h->stream));
@@ -1052,6 +1058,13 @@ int icar_adv_bind_species(void *handle, int idx, float *host, int storage, int p
         return set_err(h, ICAR_ADV_ERANGE, "pow2_exp %d out of range", pow2_exp);
     SpeciesSlot &s = h->sp[idx];
     const size_t elem = storage == ICAR_ADV_STORE_F32 ? 4 : 2;
+    // a recorded run holds the old buffers and species list: record anew
+    h->bind_gen++;
+    if (h->gexec) {
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipGraphExecDestroy(h->gexec);
+        h->gexec = nullptr;
+    }
     if (s.dev && s.storage != storage) {
         (void)hipFree(s.dev);
         s.dev = nullptr;
@@ -1626,11 +1639,10 @@ int icar_adv_snapshot(void *handle, unsigned long long mask)
         }
         h->snap_cap = total;
     }
-    if (!h->copy_stream) {
-        HIPCHK(h, hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
-        HIPCHK(h, hipEventCreateWithFlags(&h->snap_ready, hipEventDisableTiming));
-        HIPCHK(h, hipEventCreateWithFlags(&h->snap_done, hipEventDisableTiming));
-    }
+    // each created on first use (a failed creation is retried by the next call)
+    if (!h->copy_stream) HIPCHK(h, hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
+    if (!h->snap_ready) HIPCHK(h, hipEventCreateWithFlags(&h->snap_ready, hipEventDisableTiming));
+    if (!h->snap_done) HIPCHK(h, hipEventCreateWithFlags(&h->snap_done, hipEventDisableTiming));
     // on the compute stream: the current state into the device staging
     size_t m = 0;
     for (int f = 0; f < ICAR_ADV_NFIELDS; f++) {
@@ -1829,6 +1841,7 @@ int icar_adv_kernel_stats(void *handle, int id, double *total_ms, long long *lau
 namespace {
 struct Ensemble {
     std::vector<Handle *> hs;
+    std::vector<unsigned> gens;  // each member's bind_gen when the batch was built
     FastBatch *batch = nullptr;
     hipStream_t stream = nullptr;
     std::string err;
@@ -1924,6 +1937,7 @@ int icar_adv_ensemble_create(int n, void *const *handles, void **ens)
         }
         (void)hipStreamSynchronize(h->stream);
         e->hs.push_back(h);
+        e->gens.push_back(h->bind_gen);
         doms.push_back({h->fast, domain_fields(h), h->sf, h->g.dx, h->o.advect_density});
     }
     std::string m;
@@ -1953,8 +1967,14 @@ int icar_adv_ensemble_run(void *ens, const float *dt, int nsteps, int bc_x, int 
     Ensemble *e = static_cast<Ensemble *>(ens);
     if (!e) return ens_err(nullptr, ICAR_ADV_EINVAL, "NULL ensemble");
     if (!dt || nsteps < 0) return ens_err(e, ICAR_ADV_EINVAL, "bad dt or nsteps");
-    for (Handle *h : e->hs) (void)hipStreamSynchronize(h->stream);
     const int n = (int)e->hs.size();
+    for (int k = 0; k < n; k++)
+        if (e->hs[k]->bind_gen != e->gens[k])
+            return ens_err(e, ICAR_ADV_EINVAL,
+                           "member %d bound a species after the ensemble was created: its "
+                           "buffers may have moved (create the ensemble again)",
+                           k);
+    for (Handle *h : e->hs) (void)hipStreamSynchronize(h->stream);
     if (!use_graph || e->prof || nsteps == 0) return ens_steps(e, dt, nsteps, bc_x, bc_y);
     const int p0 = fast_parity(e->hs[0]->fast);
     bool hit = e->gexec && e->g_steps == nsteps && e->g_bcx == bc_x && e->g_bcy == bc_y &&

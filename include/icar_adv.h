@@ -303,7 +303,8 @@ int icar_adv_launch_config(void *handle, int *corr_cfg, int *ksplit);
 
 /* Records the nsteps loop of icar_adv_run in a HIP graph on its first call
  * and replays it on later calls with the same dt, nsteps and boundary
- * conditions (not while profiling or with debug set). */
+ * conditions (not while profiling or with debug set).  Binding a species
+ * drops the recording; the next run records anew. */
 int icar_adv_set_graph(void *handle, int enable);
 
 /* ---- Ensembles ------------------------------------------------------------
@@ -312,7 +313,10 @@ int icar_adv_set_graph(void *handle, int enable);
  * pass covering every (domain, tile, species), the step loop optionally
  * recorded in a HIP graph.  All handles need kernel_variant 1, RK3 off and
  * the same mpdata_order and MPDATA_FCT; the ensemble uses their device
- * state, so call icar_adv_ensemble_sync before using a member handle. */
+ * state, so call icar_adv_ensemble_sync before using a member handle.
+ * Members must outlive the ensemble.  Binding a member's species again (which
+ * may move its device buffers) makes icar_adv_ensemble_run return
+ * ICAR_ADV_EINVAL until the ensemble is created again. */
 int icar_adv_ensemble_create(int n, void *const *handles, void **ens);
 int icar_adv_ensemble_destroy(void *ens);
 /* dt holds one time step per domain; bc_x/bc_y as icar_adv_apply_bc. */

@@ -1,0 +1,112 @@
+"""Binding species after a run was recorded in a HIP graph, or after an
+ensemble was built: a binding may move the species' device buffer (a new
+storage format) and the fused path rebuilds its tables, so a recorded run is
+dropped and recorded again, and an ensemble refuses to run until it is built
+again."""
+import copy
+import ctypes
+
+import numpy as np
+import pytest
+
+from hicar_amd import _lib, synthetic
+from hicar_amd.advection import Advection, Ensemble
+from hicar_amd.domain import Domain
+
+pytestmark = pytest.mark.gpu
+
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def bind(st, idx, arr, storage=_lib.STORE_F32, e=0):
+    """icar_adv_bind_species on an existing handle (the Python class binds
+    only at construction)."""
+    st._check(st.lib.icar_adv_bind_species(st.h, idx, arr.ctypes.data_as(_fp), storage, e))
+    st._keep.append(arr)
+
+
+def small_case():
+    case = synthetic.config2(nx=40, ny=30, nz=8, steps=2, nspecies=3)
+    case.options.kernel_variant = 1
+    return case
+
+
+def test_binding_after_a_recorded_run_records_it_again():
+    case = small_case()
+    d, o = case.domain, case.options
+    names = list(d.species)
+    r = np.random.default_rng(23)
+    tracer = r.uniform(0.5, 1.5, d.shape).astype(np.float32)
+    tracer0 = tracer.copy()
+    st = Advection(d, o)
+    try:
+        st.set_graph(True)
+        st.run(case.dt, 2)  # recorded with three FP32 species
+        st.download()
+        # species 1 moves to BF16 storage (a new device buffer), a fourth is added
+        e1 = synthetic.pow2_scale(d.species[names[1]], _lib.STORE_BF16)
+        bind(st, 1, d.species[names[1]], _lib.STORE_BF16, e1)
+        bind(st, 3, tracer)
+        st.upload(fields=False)
+        # the same host state, bound that way from the start, run eagerly
+        d2 = Domain(d.nx, d.ny, d.nz, d.dx, halo=d.halo)
+        for n in d.FIELDS:
+            getattr(d2, n)[...] = getattr(d, n)
+        o2 = copy.deepcopy(o)
+        o2.vars_to_advect = names + ["tracer"]
+        d2.set_var_lists(o2, {**{n: a.copy() for n, a in d.species.items()},
+                              "tracer": tracer.copy()})
+        d2.storage[names[1]] = (_lib.STORE_BF16, e1)
+        assert list(d2.species) == names + ["tracer"]
+        st.run(case.dt, 2)  # same dt and steps: a stale recording would be replayed here
+        st.download()
+        ref = Advection(d2, o2)
+        try:
+            ref.run(case.dt, 2)
+            ref.download()
+        finally:
+            ref.close()
+    finally:
+        st.close()
+    for n in names:
+        np.testing.assert_array_equal(d.species[n].view(np.uint32),
+                                      d2.species[n].view(np.uint32), err_msg=n)
+    np.testing.assert_array_equal(tracer.view(np.uint32), d2.species["tracer"].view(np.uint32))
+    assert not np.array_equal(tracer, tracer0)  # the added species was advanced
+
+
+def test_ensemble_refuses_to_run_after_a_member_binds_again():
+    cases = [small_case(), synthetic.config2(nx=30, ny=26, nz=6, steps=2, nspecies=3)]
+    cases[1].options.kernel_variant = 1
+    members = [Advection(c.domain, c.options) for c in cases]
+    try:
+        ens = Ensemble(members)
+        try:
+            dts = [c.dt for c in cases]
+            ens.run(dts, 1)
+            ens.sync()
+            d = cases[0].domain
+            name = list(d.species)[1]
+            members[0].download()
+            bind(members[0], 1, d.species[name], _lib.STORE_BF16,
+                 synthetic.pow2_scale(d.species[name], _lib.STORE_BF16))
+            with pytest.raises(_lib.IcarAdvError) as err:
+                ens.run(dts, 1)
+            assert err.value.code == _lib.EINVAL and "bound a species" in str(err.value)
+        finally:
+            ens.close()
+        members[0].upload(fields=False)
+        ens = Ensemble(members)  # built again over the new buffers
+        try:
+            ens.run(dts, 1, graph=True)
+            ens.sync()
+        finally:
+            ens.close()
+        for m in members:
+            m.download()
+        for c in cases:
+            for n, a in c.domain.species.items():
+                assert np.isfinite(a).all(), n
+    finally:
+        for m in members:
+            m.close()
