@@ -1458,6 +1458,13 @@ int icar_adv_apply_forcing(void *handle, double dt)
     if (rc) return rc;
     if (!h->d_filter)
         return set_err(h, ICAR_ADV_EUNBOUND, "icar_adv_set_relax_filter has not been called");
+    // the forcing kernel updates FP32 species: a forced species bound again
+    // with 16-bit storage is refused before anything is launched
+    for (int s = 0; s < h->nspecies; s++)
+        if (h->sp[s].bound && h->sp[s].fhi && h->sp[s].storage != ICAR_ADV_STORE_F32)
+            return set_err(h, ICAR_ADV_EINVAL,
+                           "species %d has lateral forcing but 16-bit storage (forcing needs FP32)",
+                           s);
     for (int s = 0; s < h->nspecies; s++) {
         SpeciesSlot &sl = h->sp[s];
         if (!sl.bound || !sl.fhi) continue;

@@ -110,3 +110,42 @@ def test_ensemble_refuses_to_run_after_a_member_binds_again():
     finally:
         for m in members:
             m.close()
+
+
+def test_forcing_refused_after_a_forced_species_moves_to_16_bit():
+    """Lateral forcing updates FP32 species; a forced species bound again in
+    BF16 makes icar_adv_apply_forcing return EINVAL, before any launch (the
+    other species' forcing is not applied either), and binding it back to
+    FP32 lets the forcing run again."""
+    case = small_case()
+    d, o = case.domain, case.options
+    names = list(d.species)
+    r = np.random.default_rng(29)
+    f = r.uniform(0.5, 2.0, d.shape).astype(np.float32)
+    t = r.uniform(-1e-4, 1e-4, d.shape).astype(np.float32)
+    st = Advection(d, o)
+    try:
+        st.set_relax_filter(1, 1, 1, 1, halo=2)
+        st.set_forcing(names[0], f, t)
+        st.set_forcing(names[2], f, t)
+        st.apply_forcing(10.0)
+        st.download()
+        before = {n: a.copy() for n, a in d.species.items()}
+        bind(st, 0, d.species[names[0]], _lib.STORE_BF16,
+             synthetic.pow2_scale(d.species[names[0]], _lib.STORE_BF16))
+        st.upload(fields=False)
+        st.download()
+        held = {n: a.copy() for n, a in d.species.items()}
+        with pytest.raises(_lib.IcarAdvError) as err:
+            st.apply_forcing(10.0)
+        assert err.value.code == _lib.EINVAL and "16-bit" in str(err.value)
+        st.download()
+        for n in names:  # nothing ran
+            np.testing.assert_array_equal(d.species[n], held[n], err_msg=n)
+        bind(st, 0, d.species[names[0]])  # FP32 again
+        st.upload(fields=False)
+        st.apply_forcing(10.0)
+        st.download()
+        assert not np.array_equal(d.species[names[2]], before[names[2]])
+    finally:
+        st.close()
