@@ -930,6 +930,14 @@ int icar_adv_create(const icar_adv_grid *grid, const icar_adv_opts *opts, void *
                        "grid needs at least 3 cells in x, y and z and dx > 0 (got %d x %d x %d)",
                        grid->ime - grid->ims + 1, grid->jme - grid->jms + 1,
                        grid->kme - grid->kms + 1);
+    // the tile (its..ite, jts..jte) inside the memory bounds: the halo
+    // strips, the standard scheme and the dt bound address it
+    if (grid->its < grid->ims || grid->ite > grid->ime || grid->its > grid->ite ||
+        grid->jts < grid->jms || grid->jte > grid->jme || grid->jts > grid->jte)
+        return set_err(nullptr, ICAR_ADV_ESHAPE,
+                       "tile %d..%d x %d..%d must lie inside the memory bounds %d..%d x %d..%d",
+                       grid->its, grid->ite, grid->jts, grid->jte, grid->ims, grid->ime, grid->jms,
+                       grid->jme);
     if (opts->mpdata_order < 0 || opts->mpdata_order > 16)
         return set_err(nullptr, ICAR_ADV_EINVAL, "mpdata_order %d out of range [0,16]",
                        opts->mpdata_order);

@@ -37,6 +37,20 @@ def test_create_rejects_small_grid_without_device():
     assert b"at least 3" in lib.icar_adv_last_error(None)
 
 
+@pytest.mark.parametrize("tile", [(0, 6, 3, 4), (3, 9, 3, 4), (3, 6, 0, 4), (3, 6, 3, 7),
+                                  (6, 3, 3, 4), (3, 6, 4, 3)])
+def test_create_rejects_a_tile_outside_memory_without_device(tile):
+    """its..ite x jts..jte must lie inside ims..ime x jms..jme (here 1..8 x
+    1..6): the halo strips address it.  Checked before any device call."""
+    lib = _lib.load()
+    g = _lib.Grid(1, 8, 1, 5, 1, 6, *tile, 1000.0)
+    o = _lib.Opts(2, 1, 0, 0, 0, 0)
+    h = ctypes.c_void_p()
+    rc = lib.icar_adv_create(ctypes.byref(g), ctypes.byref(o), ctypes.byref(h))
+    assert rc == -2 and not h.value
+    assert b"inside the memory bounds" in lib.icar_adv_last_error(None)
+
+
 @pytest.mark.parametrize("field,val", [("mpdata_order", -1), ("kernel_variant", 7)])
 def test_create_rejects_bad_options(field, val):
     lib = _lib.load()
