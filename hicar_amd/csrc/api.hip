@@ -930,10 +930,13 @@ int icar_adv_create(const icar_adv_grid *grid, const icar_adv_opts *opts, void *
                        "grid needs at least 3 cells in x, y and z and dx > 0 (got %d x %d x %d)",
                        grid->ime - grid->ims + 1, grid->jme - grid->jms + 1,
                        grid->kme - grid->kms + 1);
-    // the tile (its..ite, jts..jte) inside the memory bounds: the halo
-    // strips, the standard scheme and the dt bound address it
-    if (grid->its < grid->ims || grid->ite > grid->ime || grid->its > grid->ite ||
-        grid->jts < grid->jms || grid->jte > grid->jme || grid->jts > grid->jte)
+    // the tile bounds inside the memory bounds: the halo strips, the standard
+    // scheme and the dt bound address the tile.  An empty tile (its > ite, as
+    // a 3-cell-wide domain with a halo of 2 has) is allowed: MPDATA does not
+    // use it, and the calls that do refuse it.
+    if (grid->its < grid->ims || grid->its > grid->ime || grid->ite < grid->ims ||
+        grid->ite > grid->ime || grid->jts < grid->jms || grid->jts > grid->jme ||
+        grid->jte < grid->jms || grid->jte > grid->jme)
         return set_err(nullptr, ICAR_ADV_ESHAPE,
                        "tile %d..%d x %d..%d must lie inside the memory bounds %d..%d x %d..%d",
                        grid->its, grid->ite, grid->jts, grid->jte, grid->ims, grid->ime, grid->jms,
@@ -1936,6 +1939,13 @@ int icar_adv_ensemble_create(int n, void *const *handles, void **ens)
             ens_free(e);
             return ens_err(nullptr, ICAR_ADV_EINVAL, "NULL handle %d", k);
         }
+        // one launch updates every member: a domain listed twice would be
+        // advanced twice over the same buffers at once
+        for (int q = 0; q < k; q++)
+            if (handles[q] == handles[k]) {
+                ens_free(e);
+                return ens_err(nullptr, ICAR_ADV_EINVAL, "handle %d repeats handle %d", k, q);
+            }
         if (h->is_std || h->o.kernel_variant != 1 || h->o.rk3 || h->o.mpdata_order < 1 ||
             h->nspecies == 0 || require_bound(h) != 0) {
             ens_free(e);

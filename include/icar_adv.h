@@ -140,8 +140,9 @@ enum {
 #define ICAR_ADV_MASK_ALL_SPECIES 0xFFFFFFFFFFFF0000ULL
 
 /* = adv_init/mpdata_init (advection_driver.f90:26-44, adv_mpdata.f90:415-442).
- * Validates bounds (nx, ny, nz >= 3; the tile its..ite x jts..jte inside
- * ims..ime x jms..jme, ICAR_ADV_ESHAPE otherwise) and allocates the device mirrors and
+ * Validates bounds (nx, ny, nz >= 3; its, ite, jts, jte inside ims..ime x
+ * jms..jme, ICAR_ADV_ESHAPE otherwise; an empty tile is allowed) and
+ * allocates the device mirrors and
  * work space on the current HIP device.  Unlike mpdata_init it keeps its
  * state in the handle, so several domains can coexist in one process. */
 int icar_adv_create(const icar_adv_grid *grid, const icar_adv_opts *opts, void **handle);

@@ -38,10 +38,11 @@ def test_create_rejects_small_grid_without_device():
 
 
 @pytest.mark.parametrize("tile", [(0, 6, 3, 4), (3, 9, 3, 4), (3, 6, 0, 4), (3, 6, 3, 7),
-                                  (6, 3, 3, 4), (3, 6, 4, 3)])
+                                  (9, 6, 3, 4), (3, 6, 3, 0)])
 def test_create_rejects_a_tile_outside_memory_without_device(tile):
-    """its..ite x jts..jte must lie inside ims..ime x jms..jme (here 1..8 x
-    1..6): the halo strips address it.  Checked before any device call."""
+    """its, ite, jts and jte must lie inside ims..ime x jms..jme (here 1..8 x
+    1..6): the halo strips address the tile.  Checked before any device
+    call."""
     lib = _lib.load()
     g = _lib.Grid(1, 8, 1, 5, 1, 6, *tile, 1000.0)
     o = _lib.Opts(2, 1, 0, 0, 0, 0)
@@ -49,6 +50,20 @@ def test_create_rejects_a_tile_outside_memory_without_device(tile):
     rc = lib.icar_adv_create(ctypes.byref(g), ctypes.byref(o), ctypes.byref(h))
     assert rc == -2 and not h.value
     assert b"inside the memory bounds" in lib.icar_adv_last_error(None)
+
+
+def test_create_accepts_an_empty_tile_inside_memory_without_device():
+    """A 3-cell-wide domain with a halo of 2 has its = 3 > ite = 1 (the
+    reference's test_mpdata domain): allowed; without a device create then
+    fails only at the device."""
+    lib = _lib.load()
+    g = _lib.Grid(1, 3, 1, 5, 1, 100, 3, 1, 3, 98, 1000.0)
+    o = _lib.Opts(2, 1, 0, 0, 0, 0)
+    h = ctypes.c_void_p()
+    rc = lib.icar_adv_create(ctypes.byref(g), ctypes.byref(o), ctypes.byref(h))
+    assert b"inside the memory bounds" not in (lib.icar_adv_last_error(None) or b"")
+    if rc == 0:
+        lib.icar_adv_destroy(h)
 
 
 @pytest.mark.parametrize("field,val", [("mpdata_order", -1), ("kernel_variant", 7)])

@@ -78,3 +78,16 @@ def test_graph_exact_path():
         np.testing.assert_array_equal(case.domain.species["qv"], eager["qv"])
     finally:
         st.close()
+
+
+def test_ensemble_rejects_a_handle_listed_twice():
+    from hicar_amd import _lib
+    cases = small_ensemble(n=2, steps=1)
+    members = [MpdataAdvection(c.domain, c.options) for c in cases]
+    try:
+        with pytest.raises(_lib.IcarAdvError) as err:
+            Ensemble([members[0], members[1], members[0]])
+        assert err.value.code == _lib.EINVAL and "repeats" in str(err.value)
+    finally:
+        for m in members:
+            m.close()
