@@ -293,3 +293,25 @@ def test_std_run_with_boundary_conditions_matches_oracle(oracle32, bc_x, bc_y, g
         st.close()
     for n in cur:
         np.testing.assert_array_equal(d.species[n], cur[n], err_msg=n)
+
+
+@pytest.mark.parametrize("h,v,fc,rk3", [(5, 3, 1, True), (3, 5, 0, True), (1, 3, 1, False)])
+def test_std_without_advect_density_matches_oracle(oracle32, h, v, fc, rk3):
+    """advect_density off (the reference's default): rho = 1 in the mass
+    fluxes, in rho_i and in the divisors J*rho and dz*J*rho
+    (advect.f90:444-492, :304-309); two steps bit for bit."""
+    case = std_case(h, v, fc, rk3, nx=38, ny=31, nz=8)
+    d = case.domain
+    case.options.parameters.advect_density = False
+    init = {n: a.copy() for n, a in d.species.items()}
+    ref = oracle_steps(oracle32, case, 2)
+    st = Advection(d, case.options)
+    try:
+        st.advect(case.dt)
+        st.advect(case.dt)
+        st.download()
+    finally:
+        st.close()
+    for n in ref:
+        np.testing.assert_array_equal(d.species[n], ref[n], err_msg=n)
+        assert not np.array_equal(ref[n], init[n]), n
