@@ -1572,9 +1572,26 @@ int icar_adv_halo_exchange(void *from, int side, void *to)
                        "neighbours do not match: strip %d x %d x %d (%d species) against %d x %d x "
                        "%d (%d species)",
                        snd.ni, a->d.nz, snd.nj, hs.n, rcv.ni, b->d.nz, rcv.nj, hb.n);
+    // the strips carry stored values: both sides need each species in the
+    // same format (FP32, BF16 or FP16; the rounding kind does not matter) and,
+    // for 16-bit formats, with the same 2^e scale
+    auto format = [](int st) {
+        return st == ICAR_ADV_STORE_F32 ? 0
+               : (st == ICAR_ADV_STORE_BF16 || st == ICAR_ADV_STORE_BF16_SR) ? 1
+                                                                              : 2;
+    };
+    std::vector<const SpeciesSlot *> sa, sb;
+    for (int s = 0; s < a->nspecies; s++)
+        if (a->sp[s].bound) sa.push_back(&a->sp[s]);
+    for (int s = 0; s < b->nspecies; s++)
+        if (b->sp[s].bound) sb.push_back(&b->sp[s]);
     for (int m = 0; m < hs.n; m++) {
-        if (hs.esize[m] != hb.esize[m])
-            return set_err(a, ICAR_ADV_ESHAPE, "species %d is stored differently on the two sides", m);
+        const int fa = format(sa[m]->storage), fb = format(sb[m]->storage);
+        if (fa != fb || (fa != 0 && sa[m]->pow2 != sb[m]->pow2))
+            return set_err(a, ICAR_ADV_ESHAPE,
+                           "species %d is stored differently on the two sides (format %d, 2^%d "
+                           "against format %d, 2^%d)",
+                           m, fa, sa[m]->pow2, fb, sb[m]->pow2);
         hs.b[m] = hb.b[m];
     }
     hs.ni = snd.ni;

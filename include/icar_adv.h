@@ -238,7 +238,10 @@ int icar_adv_compute_dt(void *handle, float cfl, int cfl_strictness, const float
  *                         (subdomains that share a GPU); ordered after the
  *                         sender's queued work, and the sender's later work
  *                         after the copy
- * Both sides need the same species, storage kinds and strip shape. */
+ * The strips carry the stored values: both sides need the same species, the
+ * same storage format per species (FP32, BF16 or FP16) with the same 2^e
+ * scale, and the same strip shape.  icar_adv_halo_exchange checks this
+ * (ICAR_ADV_ESHAPE); for pack and unpack it is the caller's contract. */
 int icar_adv_halo_bytes(void *handle, int side, unsigned long long *bytes);
 int icar_adv_halo_pack(void *handle, int side, void *dev_buf, unsigned long long bytes);
 int icar_adv_halo_unpack(void *handle, int side, const void *dev_buf, unsigned long long bytes);

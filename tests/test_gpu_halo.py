@@ -196,3 +196,57 @@ def test_two_images_exchange_and_advance_like_the_oracle(oracle32, scheme):
     for n in qw:
         np.testing.assert_array_equal(west.species[n], qw[n], err_msg=n)
         np.testing.assert_array_equal(east.species[n], qe[n], err_msg=n)
+
+
+def mpdata_images(storage_w, e_w, storage_e, e_e):
+    """West and east images of an MPDATA domain (fused path) whose species
+    qc is stored in 16 bits with the given formats and 2^e scales."""
+    case = synthetic.config2(nx=44, ny=30, nz=6, steps=1, nspecies=3)
+    case.options.kernel_variant = 1
+    west, east = split(case, 22)
+    west.storage["qc"] = (storage_w, e_w)
+    east.storage["qc"] = (storage_e, e_e)
+    return case, west, east
+
+
+def test_16bit_species_exchange_their_stored_values():
+    d0 = synthetic.config2(nx=44, ny=30, nz=6, steps=1, nspecies=3).domain
+    e = synthetic.pow2_scale(d0.species["qc"], _lib.STORE_BF16)
+    case, west, east = mpdata_images(_lib.STORE_BF16, e, _lib.STORE_BF16_SR, e)
+    sw, se = Advection(west, case.options), Advection(east, case.options)
+    try:
+        sw.download()  # the values as stored (qc rounded to BF16)
+        se.download()
+        qw = {n: a.copy() for n, a in west.species.items()}
+        qe = {n: a.copy() for n, a in east.species.items()}
+        sw.halo_exchange(_lib.EAST, se)
+        se.halo_exchange(_lib.WEST, sw)
+        sw.sync()
+        se.sync()
+        sw.download()
+        se.download()
+    finally:
+        sw.close()
+        se.close()
+    exchange_x(west, east, qw, qe)
+    for n in qw:
+        np.testing.assert_array_equal(west.species[n], qw[n], err_msg=n)
+        np.testing.assert_array_equal(east.species[n], qe[n], err_msg=n)
+
+
+@pytest.mark.parametrize("other", ["scale", "format"])
+def test_exchange_refuses_species_stored_differently(other):
+    d0 = synthetic.config2(nx=44, ny=30, nz=6, steps=1, nspecies=3).domain
+    e = synthetic.pow2_scale(d0.species["qc"], _lib.STORE_BF16)
+    if other == "scale":
+        case, west, east = mpdata_images(_lib.STORE_BF16, e, _lib.STORE_BF16, e + 1)
+    else:
+        case, west, east = mpdata_images(_lib.STORE_BF16, e, _lib.STORE_F16, e)
+    sw, se = Advection(west, case.options), Advection(east, case.options)
+    try:
+        with pytest.raises(_lib.IcarAdvError) as err:
+            sw.halo_exchange(_lib.EAST, se)
+        assert err.value.code == _lib.ESHAPE and "stored differently" in str(err.value)
+    finally:
+        sw.close()
+        se.close()
