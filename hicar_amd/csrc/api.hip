@@ -188,10 +188,9 @@ struct Handle {
     };
     std::vector<SnapItem> snap_items;
     // upload / download through pinned staging: two chunks (the DMA of one
-    // overlaps the CPU copy of the other), their events and the copy pool
+    // overlaps the CPU copy of the other) and their events
     char *xfer_pin[2] = {};
     hipEvent_t xfer_ev[2] = {};
-    std::unique_ptr<CopyPool> xfer_pool;
     bool xfer_failed = false;  // pinned staging unavailable: the runtime's copies
     // profiling
     bool prof = false;
@@ -345,7 +344,6 @@ void free_all(Handle *h)
         if (h->xfer_pin[b]) (void)hipHostFree(h->xfer_pin[b]);
         if (h->xfer_ev[b]) (void)hipEventDestroy(h->xfer_ev[b]);
     }
-    h->xfer_pool.reset();
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     fast_batch_destroy(h->batch);
     fast_destroy(h->fast);
@@ -759,26 +757,35 @@ int xfer_threads()
     return n;
 }
 
-// The copy pool, created on first use; nullptr when its threads cannot be
-// created (no exception leaves the C-ABI: the copies then run on the
-// calling thread).
-CopyPool *copy_pool(Handle *h)
+// The copy pool, one for the process: host copies are bound by host memory
+// bandwidth, so the handles share it, one copy at a time (a copy holds it
+// while it runs).  Created on first use; when its threads cannot be created
+// (no exception leaves the C-ABI) the copies run on the calling thread.
+struct SharedPool {
+    std::mutex use;
+    std::unique_ptr<CopyPool> pool;
+    bool failed = false;
+};
+
+SharedPool &shared_pool()
 {
-    if (!h->xfer_pool && !h->xfer_failed && xfer_threads() > 0) {
-        try {
-            h->xfer_pool.reset(new CopyPool(xfer_threads()));
-        } catch (...) {
-            h->xfer_failed = true;
-        }
-    }
-    return h->xfer_pool.get();
+    static SharedPool p;
+    return p;
 }
 
-void host_copy(Handle *h, void *dst, const void *src, size_t n)
+void host_copy(Handle *, void *dst, const void *src, size_t n)
 {
-    CopyPool *pool = copy_pool(h);
-    if (pool)
-        pool->copy(dst, src, n);
+    SharedPool &sp = shared_pool();
+    std::lock_guard<std::mutex> l(sp.use);
+    if (!sp.pool && !sp.failed && xfer_threads() > 0) {
+        try {
+            sp.pool.reset(new CopyPool(xfer_threads()));
+        } catch (...) {
+            sp.failed = true;
+        }
+    }
+    if (sp.pool)
+        sp.pool->copy(dst, src, n);
     else
         memcpy(dst, src, n);
 }
@@ -797,8 +804,7 @@ bool xfer_ready(Handle *h)
             return false;
         }
     }
-    copy_pool(h);
-    return !h->xfer_failed;
+    return true;
 }
 
 size_t xfer_bytes(const std::vector<XferItem> &items)
