@@ -15,7 +15,11 @@ dropped term, a wrong sign or index or a transposed operand fails one of them:
                       cross terms on linear fields; the row guards
   fct (a7)            the FCT bound property (result inside the local
                       min/max of q0 and q1) incl. the w-column ends; the
-                      step-function scenario of test_mpdata.f90:51-111
+                      x/y boundary cells left unlimited (fin = fout = 0, so
+                      beta = room/1e-15) against the w-column ends limited
+                      by their one face, and a boundary cell at its own
+                      minimum zeroing its outflow; the step-function
+                      scenario of test_mpdata.f90:51-111
   advect3d (a4)       order 1 == upwind; 1-D order 2 == textbook MPDATA
                       with the 0.5 damping and the dz rescaling of w (FP64
                       NumPy restatement); order 3 == order 2 plus one pass
@@ -388,6 +392,68 @@ def test_fct_leaves_small_fluxes_alone(orc):
     U2 = np.full(n - 1, 1e-3)
     lim = orc.fct_core(q1, q0, U2, False)
     np.testing.assert_array_equal(lim[1:-1], orc.arr(U2)[1:-1])
+
+
+def _boundary_case(end):
+    """A 6-cell row where only the end cell's own bound could limit the end
+    face.  The end cell sits 0.1 above its local minimum and sends 0.5 through
+    the end face; the interior cell next to it has 0.6 of room for the 0.5 it
+    receives (beta 1.2).  ``end`` 0 is the row start (flow +), 1 the row end
+    (flow -, the mirror image)."""
+    q1 = np.array([1.0, 0.9, 1.5, 1.5, 1.5, 1.5])
+    U2 = np.array([0.5, 0.0, 0.0, 0.0, 0.0])
+    if end == 1:
+        q1, U2 = q1[::-1].copy(), -U2[::-1]
+    return q1, U2
+
+
+@pytest.mark.parametrize("end", [0, 1])
+def test_fct_boundary_cell_is_not_limited_in_x_and_y(orc, end):
+    """adv_mpdata_FCT_core.f90:79-83 and :95-99: in x and y the boundary cell
+    has fin = fout = 0 ("no flux limitations to the boundary cell"), so its
+    beta is (q1 - qmin) / 1e-15, about 1e14, and the end face keeps its full
+    antidiffusive velocity even though the 0.5 it carries out of the end cell
+    is five times that cell's 0.1 of room above its minimum."""
+    q1, U2 = _boundary_case(end)
+    lim = orc.fct_core(q1, q1, U2, False)
+    face = 0 if end == 0 else -1
+    assert lim[face] == orc.arr(U2)[face]
+    np.testing.assert_array_equal(lim, orc.arr(U2))
+
+
+@pytest.mark.parametrize("end", [0, 1])
+def test_fct_boundary_cell_is_limited_in_w(orc, end):
+    """adv_mpdata_FCT_core.f90:76-78 and :92-94: a w column's end cell counts
+    its one face (fin = -min(0,f), fout = max(0,f)), so the same row limits
+    the end face to the end cell's room: factor (q1 - qmin)/(0.5 + 1e-15),
+    about 0.2, from the end cell (the interior cell's beta is 1.2)."""
+    q1, U2 = _boundary_case(end)
+    lim = orc.fct_core(q1, q1, U2, True).astype(np.float64)
+    q = orc.arr(q1).astype(np.float64)
+    face = 0 if end == 0 else -1
+    cell, other = (0, 1) if end == 0 else (-1, -2)
+    room = q[cell] - q[other]                       # q1 - qmin of the end cell
+    assert 0.09 < room < 0.11
+    exp = room / (0.5 + 1e-15) * orc.arr(U2).astype(np.float64)[face]
+    np.testing.assert_allclose(lim[face], exp, rtol=1e-6)
+    np.testing.assert_array_equal(lim[1:-1] if end == 0 else lim[:-1],
+                                  orc.arr(U2)[1:-1] if end == 0 else orc.arr(U2)[:-1])
+
+
+@pytest.mark.parametrize("end", [0, 1])
+def test_fct_boundary_cell_at_its_minimum_stops_its_outflow(orc, end):
+    """The other side of fin = fout = 0 for x and y: a boundary cell whose q1
+    is its own local minimum has beta_out = 0 / 1e-15 = 0, so the face out of
+    it is zeroed (factor min(1, beta_in, 0) = 0), however much room the cell
+    beyond it has.  A restatement that skipped the boundary cell's beta
+    instead would keep the face."""
+    q1 = np.array([0.9, 1.0, 1.5, 1.5, 1.5, 1.5])
+    U2 = np.array([0.5, 0.0, 0.0, 0.0, 0.0])
+    if end == 1:
+        q1, U2 = q1[::-1].copy(), -U2[::-1]
+    lim = orc.fct_core(q1, q1, U2, False)
+    face = 0 if end == 0 else -1
+    assert lim[face] == 0.0
 
 
 def run_test_mpdata(orc, fct, loops=10):

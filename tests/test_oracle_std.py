@@ -331,3 +331,86 @@ def test_top_quirk_reaches_two_levels_per_step(oracle64):
             assert a[:, :d.nz - (2 * s - 1)].min() >= 0.0, (s, n)
         reached.append(min(k for a in q.values() for k in np.where(a.min(axis=(0, 2)) < 0)[0]))
     assert reached[0] == d.nz - 1 and reached[1] == d.nz - 3
+
+
+# -- the limiter's neighbour bounds (adv_fluxcorr.f90:99-121) ---------------------
+# Dyadic winds and values make every upwind flux exact in FP32 and FP64, so the
+# expected fluxes below are closed forms, checked on both builds.
+@pytest.fixture(params=["f32", "f64"])
+def oracle(request, oracle32, oracle64):
+    return oracle32 if request.param == "f32" else oracle64
+
+
+def _bound_case(axis):
+    """A tracer of 1 with one cell of 5, in a wind of 0.25 that converges on
+    the cell just before it along ``axis`` (0 = x, 1 = y): cell c0 has inflow
+    through both faces, the one from the west (south) carrying 1 and the one
+    from the east (north) carrying the 5.  The given high-order flux on the
+    face between c0 and the 5-cell is -2.  Returns the flux_corr inputs and
+    the index of that face."""
+    n_along, n_across, nz, c0 = 12, 8, 4, 5
+    nx, ny = (n_along, n_across) if axis == 0 else (n_across, n_along)
+    q = np.ones((ny, nz, nx))
+    u = np.zeros((ny, nz, nx + 1))
+    v = np.zeros((ny + 1, nz, nx))
+    fx, fy = np.zeros_like(u), np.zeros_like(v)
+    if axis == 0:
+        q[:, :, c0 + 1] = 5.0
+        u[:, :, :c0 + 1], u[:, :, c0 + 1:] = 0.25, -0.25
+        fx[:, :, c0 + 1] = -2.0
+    else:
+        q[c0 + 1] = 5.0
+        v[:c0 + 1], v[c0 + 1:] = 0.25, -0.25
+        fy[c0 + 1] = -2.0
+    w = np.zeros((ny, nz, nx))
+    fz = np.zeros((ny, nz + 1, nx))
+    one = np.ones((ny, nz, nx))
+    tile = (3, nx - 4, 3, ny - 4)
+    return (q, u, v, w, fx, fy, fz, one, one, one) + tile, c0 + 1
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_limiter_convergent_face_takes_only_the_west_neighbour(oracle, axis):
+    """adv_fluxcorr.f90:99-113: `where (u(i) > 0) ... else where (u(i+1) < 0)`.
+    A cell with inflow through both faces takes its bounds from the west
+    (south) neighbour only, so c0's qmax is 1 while its upwind-only value is
+    2.4375 (it receives the 5): scale_in(c0) = 0 and the face from the
+    5-cell keeps only its two-half-step upwind flux,
+        -0.5*0.25*5 - 0.5*0.25*(5 - 0.5*0.25*4) = -1.1875
+    (dumb_q of the 5-cell is 4.5).  Were both neighbours taken, qmax would
+    be 5 and the given -2 would pass unlimited."""
+    args, face = _bound_case(axis)
+    gx, gy, _ = oracle.std_flux_corr(*args)
+    its, ite, jts, jte = args[-4:]
+    if axis == 0:
+        got = gx[jts - 1:jte + 2, :, face]
+    else:
+        got = gy[face, :, its - 1:ite + 2]
+    np.testing.assert_array_equal(got, np.full(got.shape, -1.1875))
+
+
+def test_limiter_vertical_bounds_follow_each_faces_own_sign(oracle):
+    """adv_fluxcorr.f90:115-121: a cell takes the level below into its bounds
+    when the face below it has w > 0, and the level above when the face above
+    it has w < 0.  Level k0 = 3 holds 2 between 0.5 below and 1 above, with
+    w = -0.25 below it and +0.25 above it (outflow both ways): its bounds are
+    its own value alone, its upwind-only value is 1.125, so scale_out(k0) = 0
+    and the face above it keeps only its upwind flux,
+        0.5*0.25*2 + 0.5*0.25*(2 - 0.5) = 0.4375,
+    for the given 0.875.  Taking the level below on the sign of the face
+    above, or the level above on the sign of the face below, widens the
+    bounds and lets the given flux through."""
+    nx, ny, nz, k0 = 8, 7, 8, 3
+    q = np.ones((ny, nz, nx))
+    q[:, k0 - 1], q[:, k0] = 0.5, 2.0
+    w = np.full((ny, nz, nx), 0.25)
+    w[:, :k0] = -0.25
+    u, v = np.zeros((ny, nz, nx + 1)), np.zeros((ny + 1, nz, nx))
+    fx, fy = np.zeros_like(u), np.zeros_like(v)
+    fz = np.zeros((ny, nz + 1, nx))
+    fz[:, k0 + 1] = 0.875
+    one = np.ones((ny, nz, nx))
+    its, ite, jts, jte = 3, nx - 4, 3, ny - 4
+    _, _, gz = oracle.std_flux_corr(q, u, v, w, fx, fy, fz, one, one, one, its, ite, jts, jte)
+    got = gz[jts - 1:jte + 2, k0 + 1, its - 1:ite + 2]
+    np.testing.assert_array_equal(got, np.full(got.shape, 0.4375))
