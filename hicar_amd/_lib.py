@@ -44,7 +44,7 @@ EXPORTS = (
     "icar_adv_compute_dt", "icar_adv_set_relax_filter", "icar_adv_set_forcing",
     "icar_adv_apply_forcing", "icar_adv_halo_bytes", "icar_adv_halo_pack",
     "icar_adv_halo_unpack", "icar_adv_halo_exchange", "icar_adv_enforce_limits",
-    "icar_adv_snapshot", "icar_adv_snapshot_wait",
+    "icar_adv_snapshot", "icar_adv_snapshot_wait", "icar_adv_divergence",
 )
 
 
@@ -92,6 +92,7 @@ def load() -> ctypes.CDLL:
         "icar_adv_advect": ([vp, ctypes.c_float], ip),
         "icar_adv_mass_budget": ([vp, dp], ip),
         "icar_adv_max_courant": ([vp, dp], ip),
+        "icar_adv_divergence": ([vp, dp], ip),
         "icar_adv_last_error": ([vp], ctypes.c_char_p),
         "icar_adv_sync": ([vp], ip),
         "icar_adv_apply_bc": ([vp, ip, ip], ip),

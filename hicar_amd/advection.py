@@ -111,6 +111,15 @@ class Advection:
             self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
+    def divergence(self) -> tuple[int, float]:
+        """test_divergence (adv_mpdata.f90:445-475) on the Courant numbers of
+        the last step: the number of interior cells with |div| > 1e-3 and the
+        largest |div|."""
+        out = np.zeros(2, np.float64)
+        self._check(self.lib.icar_adv_divergence(
+            self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return int(out[0]), float(out[1])
+
     def balance_w(self) -> None:
         """balance_uvw (wind.f90:88-164) on the device copies of the winds:
         the device w becomes the continuity-balanced one; ``download(

@@ -182,6 +182,7 @@ struct Handle {
     hipStream_t copy_stream = nullptr;
     hipEvent_t snap_ready = nullptr, snap_done = nullptr;
     bool snap_pending = false;
+    unsigned long long snap_mask = 0;  // the upload/download bits of the pending snapshot
     struct SnapItem {
         float *host;
         size_t off, n;
@@ -448,12 +449,15 @@ TimedFn timed_fn(T *h)
     return [h](int id, double bytes, const std::function<void()> &fn) { timed(h, id, bytes, fn); };
 }
 
-// test_divergence (adv_mpdata.f90:500-502): the reference prints a warning
-int divergence_check(Handle *h)
+// test_divergence (adv_mpdata.f90:445-475) on the Courant numbers of the last
+// MPDATA step: the number of interior cells with |div| > 1e-3 and the largest
+// |div| (an FP32 value, exact in the double).  Synchronises.
+int divergence_of_step(Handle *h, double *cnt_out, double *max_out)
 {
     const int nb = mass_partials_blocks(h->d);
     launch_divergence(h->stream, h->d, h->sf.U, h->sf.V, h->sf.W,
                       h->dev_field[ICAR_ADV_FIELD_ADVECTION_DZ], h->d_partial);
+    HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipMemcpyAsync(h->h_partial.data(), h->d_partial, sizeof(double) * 2 * nb,
                              hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -462,6 +466,17 @@ int divergence_check(Handle *h)
         cnt += h->h_partial[2 * b];
         mx = mx > h->h_partial[2 * b + 1] ? mx : h->h_partial[2 * b + 1];
     }
+    *cnt_out = cnt;
+    *max_out = mx;
+    return 0;
+}
+
+// test_divergence (adv_mpdata.f90:500-502): the reference prints a warning
+int divergence_check(Handle *h)
+{
+    double cnt = 0, mx = 0;
+    const int rc = divergence_of_step(h, &cnt, &mx);
+    if (rc) return rc;
     if (cnt > 0)
         fprintf(stderr,
                 "icar_adv: winds are not balanced on entry to advect: %.0f cells with "
@@ -643,7 +658,7 @@ int sync_fast_species(Handle *h)
     }
     fast_batch_destroy(h->batch);  // its tables point at the old buffers
     h->batch = nullptr;
-    int rc = fast_set_species(h->fast, v.data(), (int)v.size());
+    int rc = fast_set_species(h->fast, v.data(), (int)v.size(), h->stream);
     if (rc) return set_err(h, rc, "%s", fast_error(h->fast));
     return 0;
 }
@@ -1119,6 +1134,13 @@ int icar_adv_upload(void *handle, unsigned long long mask)
     Handle *h;
     int rc = check_handle(handle, &h);
     if (rc) return rc;
+    // A pending snapshot has yet to write its host arrays.  Completing it here
+    // would overwrite what the caller wrote into them since, and completing it
+    // later would overwrite them after this upload read them: refuse instead.
+    if (h->snap_pending && (mask & h->snap_mask))
+        return set_err(h, ICAR_ADV_EINVAL,
+                       "a snapshot of these arrays is pending: icar_adv_snapshot_wait before "
+                       "uploading them");
     // FP32 arrays in one pipelined transfer; each 16-bit species through the
     // FP32 staging and the pack kernel (stream order keeps the staging's uses
     // apart)
@@ -1154,6 +1176,10 @@ int icar_adv_download(void *handle, unsigned long long mask)
 {
     Handle *h;
     int rc = check_handle(handle, &h);
+    if (rc) return rc;
+    // a pending snapshot writes the host arrays first, so that the newer
+    // values of this download are the ones that stay
+    rc = finish_snapshot(h);
     if (rc) return rc;
     std::vector<XferItem> items;
     for (int f = 0; f < ICAR_ADV_NFIELDS; f++) {
@@ -1251,6 +1277,19 @@ int icar_adv_max_courant(void *handle, double out[3])
         for (int c = 0; c < 3; c++)
             out[c] = out[c] > h->h_partial[3 * b + c] ? out[c] : h->h_partial[3 * b + c];
     return 0;
+}
+
+int icar_adv_divergence(void *handle, double out[2])
+{
+    Handle *h;
+    int rc = check_handle(handle, &h);
+    if (rc) return rc;
+    if (!out) return set_err(h, ICAR_ADV_EINVAL, "NULL output");
+    if (h->is_std)
+        return set_err(h, ICAR_ADV_EINVAL,
+                       "the divergence check belongs to MPDATA (the standard scheme has none)");
+    if (!h->have_dt) return set_err(h, ICAR_ADV_EINVAL, "advect has not been called yet");
+    return divergence_of_step(h, &out[0], &out[1]);
 }
 
 int icar_adv_balance_w(void *handle)
@@ -1587,11 +1626,17 @@ int icar_adv_halo_exchange(void *from, int side, void *to)
                                                                               : 2;
     };
     std::vector<const SpeciesSlot *> sa, sb;
+    std::vector<int> ia, ib;  // species index of each strip
     for (int s = 0; s < a->nspecies; s++)
-        if (a->sp[s].bound) sa.push_back(&a->sp[s]);
+        if (a->sp[s].bound) sa.push_back(&a->sp[s]), ia.push_back(s);
     for (int s = 0; s < b->nspecies; s++)
-        if (b->sp[s].bound) sb.push_back(&b->sp[s]);
+        if (b->sp[s].bound) sb.push_back(&b->sp[s]), ib.push_back(s);
     for (int m = 0; m < hs.n; m++) {
+        if (ia[m] != ib[m])
+            return set_err(a, ICAR_ADV_ESHAPE,
+                           "strip %d carries species %d on the sending side and species %d on "
+                           "the receiving side: both sides need the same species bound",
+                           m, ia[m], ib[m]);
         const int fa = format(sa[m]->storage), fb = format(sb[m]->storage);
         if (fa != fb || (fa != 0 && sa[m]->pow2 != sb[m]->pow2))
             return set_err(a, ICAR_ADV_ESHAPE,
@@ -1714,6 +1759,7 @@ int icar_adv_snapshot(void *handle, unsigned long long mask)
     HIPCHK(h, hipEventRecord(h->snap_done, h->copy_stream));
     h->snap_items = items;
     h->snap_pending = true;
+    h->snap_mask = mask;
     return 0;
 }
 

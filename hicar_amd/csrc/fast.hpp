@@ -18,7 +18,10 @@ struct FastState;
 
 FastState *fast_create(Dims d);
 void fast_destroy(FastState *st);
-int fast_set_species(FastState *st, const FastSpecies *sp, int n);
+// Replaces the species list.  Species that keep their home buffer keep
+// their current device values (copied home on `stream` when the current
+// state is in the second buffer); the second buffers are allocated anew.
+int fast_set_species(FastState *st, const FastSpecies *sp, int n, hipStream_t stream);
 const char *fast_error(FastState *st);
 // Device buffer holding the current state of species s (in bind order).
 void *fast_species_ptr(FastState *st, int s);

@@ -1085,8 +1085,30 @@ void fast_destroy(FastState *st)
     delete st;
 }
 
-int fast_set_species(FastState *st, const FastSpecies *sp, int n)
+int fast_set_species(FastState *st, const FastSpecies *sp, int n, hipStream_t stream)
 {
+    // At mpdata_order <= 2 the steps alternate between the two home buffers.
+    // When the current state sits in the second buffer, a species that keeps
+    // its home buffer (same buffer, same storage) gets its current values
+    // copied home first, so that binding another species leaves it as it was.
+    if (st->cur == 1) {
+        for (int s = 0; s < n; s++)
+            for (size_t o = 0; o < st->sp.size() && o < st->alt.size(); o++)
+                if (st->sp[o].dev == sp[s].dev && st->sp[o].storage == sp[s].storage &&
+                    st->alt[o]) {
+                    if (hipMemcpyAsync(sp[s].dev, st->alt[o],
+                                       st->N * kind_bytes(sp[s].storage), hipMemcpyDeviceToDevice,
+                                       stream) != hipSuccess) {
+                        st->err = "copy of a species' current state to its home buffer failed";
+                        return ICAR_ADV_ENOMEM;
+                    }
+                    break;
+                }
+        if (hipStreamSynchronize(stream) != hipSuccess) {
+            st->err = "copy of the species' current states to their home buffers failed";
+            return ICAR_ADV_ENOMEM;
+        }
+    }
     free_species_buffers(st);
     st->sp.assign(sp, sp + n);
     st->cur = 0;

@@ -184,6 +184,16 @@ int icar_adv_mass_budget(void *handle, double *per_species);
  * standard scheme's (advect.f90:444-492).  Synchronises. */
 int icar_adv_max_courant(void *handle, double out[3]);
 
+/* = test_divergence (adv_mpdata.f90:445-475), which mpdata_compute_wind
+ * runs when debug is set (:500-502), on the Courant numbers of the last
+ * advect call: out[0] = the number of interior cells (ims+1:ime-1, all k,
+ * jms+1:jme-1) whose discrete divergence |dU + dV + dW/dz| exceeds 1e-3 (the
+ * cells the reference prints a warning for), out[1] = the largest such
+ * |divergence| (an FP32 value).  With debug set advect also prints the
+ * count to stderr.  MPDATA only; needs one advect call first.
+ * Synchronises. */
+int icar_adv_divergence(void *handle, double out[2]);
+
 /* = balance_uvw(domain, options) (src/physics/wind.f90:88-164, update
  * =.false.), which the time step runs before advect when advect_density is
  * set (src/main/time_step.f90:324): recomputes the device copy of w from
@@ -259,7 +269,10 @@ int icar_adv_enforce_limits(void *handle);
  * species decoded to FP32), then into pinned host staging on a copy stream
  * that overlaps the following steps.  icar_adv_snapshot_wait blocks until
  * that copy is done and writes the values into the bound host arrays.  A
- * second snapshot, or destroy, completes a pending one first. */
+ * second snapshot, icar_adv_download or destroy completes a pending one
+ * first (so a later download's values are the ones the host arrays keep);
+ * icar_adv_upload of an array a pending snapshot covers returns
+ * ICAR_ADV_EINVAL, since the snapshot has yet to write it. */
 int icar_adv_snapshot(void *handle, unsigned long long mask);
 int icar_adv_snapshot_wait(void *handle);
 

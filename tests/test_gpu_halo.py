@@ -12,6 +12,9 @@ domain_obj.f90:573-650 and exchangeable_obj.f90:240-392), on one GPU:
     exchange (numpy slices) and steps on the two images, bit for bit, for
     the standard scheme and for MPDATA.
 """
+import copy
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -247,6 +250,30 @@ def test_exchange_refuses_species_stored_differently(other):
         with pytest.raises(_lib.IcarAdvError) as err:
             sw.halo_exchange(_lib.EAST, se)
         assert err.value.code == _lib.ESHAPE and "stored differently" in str(err.value)
+    finally:
+        sw.close()
+        se.close()
+
+
+def test_exchange_refuses_different_species_of_equal_count():
+    """The strips are paired by species: sides binding species {0, 1} and
+    {0, 2} hold the same number of strips, and the exchange refuses them
+    rather than write species 1 into species 2's halo."""
+    case = synthetic.config2(nx=44, ny=30, nz=6, steps=1, nspecies=2)
+    case.options.kernel_variant = 1
+    west, east = split(case, 22)
+    first = list(east.species)[0]
+    o1 = copy.deepcopy(case.options)
+    o1.vars_to_advect = [first]
+    east.set_var_lists(o1, {first: east.species[first]})
+    sw, se = Advection(west, case.options), Advection(east, o1)
+    extra = np.ones(east.shape, np.float32)
+    try:
+        se._check(se.lib.icar_adv_bind_species(
+            se.h, 2, extra.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _lib.STORE_F32, 0))
+        with pytest.raises(_lib.IcarAdvError) as err:
+            sw.halo_exchange(_lib.EAST, se)
+        assert err.value.code == _lib.ESHAPE and "same species" in str(err.value)
     finally:
         sw.close()
         se.close()

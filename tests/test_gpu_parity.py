@@ -139,6 +139,50 @@ def test_mass_budget_and_courant(oracle32):
         st.close()
 
 
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("balanced", [True, False])
+def test_divergence_check_through_the_abi(oracle32, variant, balanced, capfd):
+    """test_divergence (adv_mpdata.f90:445-475): the count of interior cells
+    with |div| > 1e-3 and the largest |div| of the step's Courant numbers,
+    equal to the oracle's on the same FP32 Courant numbers; with debug set
+    advect also prints the warning when the count is not zero."""
+    if balanced:  # w from discrete continuity: the reference's check stays quiet
+        case = synthetic.config1(nx=40, ny=12, nz=8, steps=1)
+    else:  # random winds: most cells exceed 1e-3
+        case = random_domain(30, 20, 7, 31, 2, True, True)
+    d, o = case.domain, case.options
+    o.kernel_variant = variant
+    o.parameters.debug = True
+    st = MpdataAdvection(d, o)
+    try:
+        with pytest.raises(Exception, match="advect has not been called"):
+            st.divergence()
+        st.advect(case.dt)
+        cnt, mx = st.divergence()
+    finally:
+        st.close()
+    U, V, W = oracle32.compute_wind(d, case.dt, o.parameters.advect_density)
+    n_ref, mx_ref = oracle32.divergence(U, V, W, oracle32.arr(d.advection_dz))
+    assert (cnt, mx) == (n_ref, mx_ref)
+    err = capfd.readouterr().err
+    if balanced:
+        assert n_ref == 0 and mx_ref < 1e-4 and "not balanced" not in err
+    else:
+        assert n_ref > (d.nx - 2) * (d.ny - 2) * d.nz // 2
+        assert f"{n_ref} cells" in err
+
+
+def test_divergence_check_refused_for_the_standard_scheme():
+    std = synthetic.config_std(nx=20, ny=16, nz=6, steps=1, nspecies=1, cfl=1.0)
+    st = MpdataAdvection(std.domain, std.options)
+    try:
+        st.advect(std.dt)
+        with pytest.raises(Exception, match="belongs to MPDATA"):
+            st.divergence()
+    finally:
+        st.close()
+
+
 def test_upload_download_roundtrip_and_shape_errors():
     case = synthetic.config1(nx=20, ny=7, nz=5, steps=1)
     d = case.domain

@@ -149,3 +149,53 @@ def test_forcing_refused_after_a_forced_species_moves_to_16_bit():
         assert not np.array_equal(d.species[names[2]], before[names[2]])
     finally:
         st.close()
+
+
+@pytest.mark.parametrize("storage", [_lib.STORE_F32, _lib.STORE_BF16])
+def test_binding_at_order_2_keeps_the_other_species_state(oracle32, storage):
+    """At mpdata_order 2 the fused path alternates between two buffers per
+    species, so after one step the current state sits in the second one.
+    Binding a new species then uploading only that species keeps the other
+    species' device state: the next step continues from it, bit for bit as a
+    handle that takes both steps without the new species (and, in FP32, as
+    the oracle's two steps), and the new species takes one step from its
+    uploaded values."""
+    case = synthetic.config2(nx=40, ny=30, nz=8, steps=2, nspecies=3)
+    case.options.kernel_variant = 1
+    case.options.adv_options.mpdata_order = 2
+    d, o = case.domain, case.options
+    names = list(d.species)
+    if storage != _lib.STORE_F32:
+        d.storage[names[1]] = (storage, synthetic.pow2_scale(d.species[names[1]], storage))
+    init = {n: a.copy() for n, a in d.species.items()}
+    r = np.random.default_rng(41)
+    tracer = r.uniform(0.5, 1.5, d.shape).astype(np.float32)
+    tracer0 = tracer.copy()
+    st = Advection(d, o)
+    try:
+        st.advect(case.dt)  # the state moves to the second buffers
+        bind(st, 3, tracer)
+        st._check(st.lib.icar_adv_upload(st.h, _lib.mask_species(3)))
+        st.advect(case.dt)
+        st.download()
+    finally:
+        st.close()
+    got = {n: d.species[n].copy() for n in names}
+    for n, a in init.items():  # the same two steps without the binding
+        d.species[n][...] = a
+    ref = Advection(d, o)
+    try:
+        ref.advect(case.dt)
+        ref.advect(case.dt)
+        ref.download()
+    finally:
+        ref.close()
+    for n in names:
+        np.testing.assert_array_equal(got[n].view(np.uint32), d.species[n].view(np.uint32),
+                                      err_msg=n)
+    fp32 = [n for n in names if n not in d.storage or d.storage[n][0] == _lib.STORE_F32]
+    orc = oracle32.run(d, o, case.dt, 2, 0, 0, {n: init[n] for n in fp32})
+    for n in fp32:
+        np.testing.assert_array_equal(got[n].view(np.uint32), orc[n].view(np.uint32), err_msg=n)
+    tr = oracle32.run(d, o, case.dt, 1, 0, 0, {"tracer": tracer0})["tracer"]
+    np.testing.assert_array_equal(tracer.view(np.uint32), tr.view(np.uint32))
