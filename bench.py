@@ -1,27 +1,35 @@
 """Benchmark of the MPDATA advection step on one MI355X.
 
-Workload (the single-domain configuration the metric is quoted on): a
-1024 x 1024 x 30 domain, 10 species (qv, theta, qc, qr, qs, qi, qg, ni, nr
-and a passive tracer), mpdata_order 3 with FCT, advect_density on, fractal
-terrain and a 15 m/s westerly with terrain-forced perturbation, w from
-continuity, dt at Courant 0.5.  A step is one advect() call (Courant numbers
-plus every species through all MPDATA passes) with all inputs resident in
-HBM.  value = cell-species updates per second = nx*ny*nz*species*steps / t.
+Headline workload (the largest configuration that fits one GPU): one
+4096 x 4096 x 50 domain (839 M cells, dx 250 m, fractal terrain with a
+3000 m ridge line, a 20 m/s jet at 8 km plus a terrain-forced perturbation,
+w from continuity, dt at Courant 0.5), 12 species (qv, theta, qc, qr, qs,
+qi, qg, ni, nr, ns, ng and a passive tracer), mpdata_order 2 with FCT,
+advect_density on.  A step is one advect() call (Courant numbers plus every
+species through all MPDATA passes) with all inputs resident in HBM.
+value = cell-species updates per second = nx*ny*nz*species*steps / t.
 
-Variants: "plain" (exact per-species kernels, the drop-in FP32 form),
-"batched" (fused species-batched kernels, FP32, bit-identical to plain) and
-"batched_bf16" (the same with the hydrometeor masses and numbers stored in
-BF16 with power-of-two scaling and stochastic rounding, qv and theta FP32).  value is the best
-bit-exact FP32 variant; the others are reported beside it, with a small
-launch-bound case (the 256x64x20 ridge domain) run eagerly and replayed
-from a HIP graph.
+Variants on that domain:
+  batched_mixed  fused species-batched kernels, qv and theta FP32 and the
+                 ten others stored in BF16 with power-of-two scaling (the
+                 value reported);
+  batched        the same kernels with every species FP32 (bit-identical to
+                 the FP32 oracle);
+  plain          one kernel per MPDATA pass and species, every species FP32:
+                 the drop-in form the mixed variant is compared with.
+The CPU baseline is the FP32 oracle (a scalar C restatement of the
+reference) on the host cores, one subdomain per core as ICAR runs its
+images, on a strip of the same domain.
+
+Beside it (unless --no-extras): the 1024 x 1024 x 30, 10-species domain at
+mpdata_order 3 + FCT in its three variants, a launch-bound 256x64x20 case
+eager and from a HIP graph, the 48-domain ensemble, the standard scheme,
+the host hand-off and the kernels next to advect.
 
 Prints one JSON line.  Options: --steps K --warmup W --gpus N (1 only)
---variant plain|batched|batched_bf16|all --no-cpu (skip the CPU baseline)
---no-small (skip the launch-bound case) --no-ensemble (skip the 48-domain
-ensemble, reported beside the single-domain value) --no-std (skip the
-standard scheme: h_order 5, v_order 3, WRF limiter, RK3 on the same domain,
-reported beside it).
+--workload c5|c2 (c2 puts the 1024x1024x30 domain in the headline line)
+--no-cpu --no-extras --no-plain, and for the c2 extras --no-small
+--no-ensemble --no-std --no-handoff --no-wind.
 """
 from __future__ import annotations
 
@@ -240,11 +248,16 @@ KERNEL_SYMBOL = {"fused_corrective_pass": "k_corr", "fused_first_pass": "k_first
                  "pseudo_velocity": "k_pseudo", "fct": "k_fct", "copy": "k_copy"}
 
 
-def read_traffic(kernel: str):
-    """HBM bytes per launch of a kernel from the committed PMC summary
-    (profiles/traffic_*.txt: lines 'kernel fetch_bytes write_bytes')."""
-    import glob
-    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "traffic_*.txt"))):
+def _traffic_file(workload: str) -> str:
+    return os.path.join(ROOT, "profiles", f"traffic_{workload}.txt")
+
+
+def read_traffic(kernel: str, workload: str = "c2_1024x1024x30"):
+    """HBM bytes per launch of a kernel from the committed PMC summary of the
+    workload (profiles/traffic_<workload>.txt: lines 'kernel fetch_bytes
+    write_bytes')."""
+    path = _traffic_file(workload)
+    if os.path.exists(path):
         for line in open(path):
             parts = line.split()
             if len(parts) == 3 and parts[0] == kernel:
@@ -270,11 +283,11 @@ def step_bytes(d) -> float:
     return float(b)
 
 
-def read_valu(kernel: str):
+def read_valu(kernel: str, workload: str = "c2_1024x1024x30"):
     """VALU wave-instructions per launch of a kernel from the committed PMC
-    summary (lines 'valu kernel count')."""
-    import glob
-    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "traffic_*.txt"))):
+    summary of the workload (lines 'valu kernel count')."""
+    path = _traffic_file(workload)
+    if os.path.exists(path):
         for line in open(path):
             parts = line.split()
             if len(parts) == 3 and parts[0] == "valu" and parts[1] == kernel:
@@ -282,78 +295,194 @@ def read_valu(kernel: str):
     return None
 
 
-def cpu_baseline(case):
+def host_cores() -> dict:
+    """Threads this process may use (the CPU share of the host, at most 16
+    on a one-GPU box) and the host's own counts (logical CPUs, physical cores
+    as lscpu lists them)."""
+    import subprocess
+    try:
+        share = len(os.sched_getaffinity(0))
+    except AttributeError:
+        share = os.cpu_count() or 1
+    phys = None
+    try:
+        r = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=10)
+        cores = {ln for ln in r.stdout.splitlines() if ln and not ln.startswith("#")}
+        phys = len(cores) or None
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"threads": max(1, min(16, share)), "logical_cpus": os.cpu_count(),
+            "physical_cores": phys}
+
+
+def _image(d, j0: int, j1: int):
+    """Rows j0..j1-1 of the domain as a subdomain of its own (an ICAR image:
+    its outer ring, here the rows next to its neighbours, is its halo)."""
+    from hicar_amd.domain import Domain
+    sub = Domain(d.nx, j1 - j0, d.nz, d.dx, halo=d.halo)
+    for n in d.FIELDS:
+        rows = j1 - j0 + (1 if n in ("v", "jacobian_v") else 0)
+        getattr(sub, n)[...] = getattr(d, n)[j0:j0 + rows]
+    return sub, {n: a[j0:j1].copy() for n, a in d.species.items()}
+
+
+def cpu_baseline(case, rows: int = None):
     """The FP32 oracle (a scalar C restatement of the reference) on the host
-    cores: one step of the full workload with one thread per species (the
-    species of advect are independent, as ICAR's images are; ctypes releases
-    the GIL, so the C code runs in parallel), and, for reference, one thread
-    on a 512 x 512 x 30 crop."""
-    import os
+    cores, parallel the way ICAR is: one image per core
+    (src/objects/grid_obj.f90:40 decomposes the domain; between halo
+    exchanges the images advance independently).  The sample is the strip
+    of the first `rows` rows (all of them by default) cut into one image of
+    rows/threads owned rows per thread, each with the MPDATA halo of 2 rows
+    on its inner sides; one advect() of every species on every image,
+    timed around the parallel map (ctypes releases the GIL, so the C code
+    runs in parallel).  The rate counts owned cells.  For reference, one
+    image alone on one thread."""
     from concurrent.futures import ThreadPoolExecutor
 
     import numpy as np
 
-    from hicar_amd import synthetic
     from oracle.oracle import Oracle
     orc = Oracle(np.float32)
     d = case.domain
-    names = list(d.species)
-    threads = min(len(names), max(1, min(16, os.cpu_count() or 1)))
-    t0 = time.perf_counter()
+    S = len(d.species)
+    hc = host_cores()
+    threads = hc["threads"]
+    rows = min(rows or d.ny, d.ny)
+    per = max(3, rows // threads)
+    images = []
+    for j0 in range(0, rows - per + 1, per):
+        a, b = max(0, j0 - 2), min(d.ny, j0 + per + 2)
+        images.append(_image(d, a, b) + (per,))
+    owned = sum(p for _, _, p in images) * d.nx * d.nz * S
+
+    def one(img):
+        sub, sp, _ = img
+        orc.advect(sub, case.options, case.dt, species=sp)
+
     with ThreadPoolExecutor(threads) as ex:
-        list(ex.map(lambda n: orc.advect(d, case.options, case.dt, species={n: d.species[n]}),
-                    names))
-    t = time.perf_counter() - t0
-    n = d.nx * d.ny * d.nz * len(names)
-    crop = synthetic.config2(nx=512, ny=512, nz=30, steps=1)
+        list(ex.map(one, images[:1]))  # warm the oracle's code and pages once
+        t0 = time.perf_counter()
+        list(ex.map(one, images))
+        t = time.perf_counter() - t0
     t1 = time.perf_counter()
-    orc.advect(crop.domain, crop.options, crop.dt)
+    one(images[0])
     t1 = time.perf_counter() - t1
-    c = crop.domain
-    n1 = c.nx * c.ny * c.nz * len(c.species)
-    return {"value": n / t, "unit": "cell-species updates/s", "cores": threads, "kind": "port",
-            "sample": f"one step of the {d.nx}x{d.ny}x{d.nz}, {len(names)}-species workload, "
-                      f"one thread per species ({t:.1f} s)",
+    n1 = images[0][2] * d.nx * d.nz * S
+    return {"value": owned / t, "unit": "cell-species updates/s", "cores": threads,
+            "kind": "port", "host": hc,
+            "sample": f"one advect() of {S} species on rows 0..{rows - 1} of the "
+                      f"{d.nx}x{d.ny}x{d.nz} domain, as {len(images)} images of {per} owned "
+                      f"rows (+2 halo rows inside), one per thread on {threads} threads "
+                      f"({t:.1f} s)",
             "single_thread": {"value": n1 / t1, "cores": 1,
-                              "sample": "512x512x30 crop, 10 species, order 3 + FCT, 1 step "
-                                        f"({t1:.1f} s)"}}
+                              "sample": f"one of those images on one thread ({t1:.1f} s)"}}
 
 
-def bench_c5(args) -> int:
-    """The 4096x4096x50 domain, 12 species (qv and theta FP32, ten BF16 with
-    power-of-two scaling), order 2 + FCT, batched kernels: one JSON line."""
-    from hicar_amd import synthetic
+def kernel_table(stats) -> dict:
+    return {k: {"avg_ms": v["total_ms"] / v["launches"], "launches": v["launches"],
+                "achieved_GBps": v["alg_bytes"] / v["total_ms"] / 1e6}
+            for k, v in stats.items()}
+
+
+def roofline_of(stats, d, units_per_launch, workload: str, kernel: str) -> dict:
+    """The HBM roofline of one kernel: the step's compulsory bytes per
+    cell-species times the cell-species one launch advances, over the live
+    HIP-event launch duration; traffic and VALU count from the committed PMC
+    summary of the workload."""
+    S = len(d.species)
+    cells = d.nx * d.ny * d.nz
+    per_unit = step_bytes(d) / (cells * S)
+    ks = stats[kernel]
+    avg_s = ks["total_ms"] / ks["launches"] / 1e3
+    achieved = per_unit * units_per_launch / avg_s
+    sym = KERNEL_SYMBOL.get(kernel, kernel)
+    out = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK, "traffic": read_traffic(sym, workload), "kernel": kernel,
+           "avg_launch_ms": avg_s * 1e3, "bytes_per_cell_species": per_unit,
+           "cell_species_per_launch": units_per_launch,
+           "kernel_own_bytes_per_launch": ks["alg_bytes"] / ks["launches"]}
+    vi = read_valu(sym, workload)
+    if vi is not None:
+        out["valu"] = {"wave_instr_per_launch": vi, "achieved": vi / avg_s / 1e9,
+                       "peak": VALU_ISSUE_PEAK / 1e9, "unit": "G wave-instr/s",
+                       "frac": vi / avg_s / VALU_ISSUE_PEAK,
+                       "lane_instr_per_cell_species": 64.0 * vi / units_per_launch}
+    return out
+
+
+def c5_line(args) -> dict:
+    """The headline: the 4096x4096x50 domain, 12 species, order 2 + FCT, in
+    its three variants (batched_mixed is the value), and the CPU baseline on
+    a strip of the same domain."""
+    from hicar_amd import _lib, synthetic
     t0 = time.perf_counter()
     case = synthetic.config5(steps=args.steps, log=log)
     gen_s = time.perf_counter() - t0
     d = case.domain
     S = len(d.species)
     cells = d.nx * d.ny * d.nz
-    log(f"C5 generated in {gen_s:.0f} s; {args.warmup} warmup + {args.steps} timed steps ...")
-    t, stats = measure(case, 1, args.steps, args.warmup)
-    lc = stats.pop("_launch_config", None)
-    b_step = step_bytes(d)
-    value = cells * S * args.steps / t
-    log(f"  C5: {value / 1e9:.2f} G cell-species/s, {1e3 * t / args.steps:.1f} ms/step")
+    mixed = dict(d.storage)
+    log(f"4096x4096x50 domain generated in {gen_s:.0f} s")
+    todo = ["batched_mixed", "batched"] + ([] if args.no_plain else ["plain"])
+    results = {}
+    for name in todo:
+        d.storage.clear()
+        if name == "batched_mixed":
+            d.storage.update(mixed)
+        log(f"variant {name}: {args.warmup} warmup + {args.steps} timed steps ...")
+        try:
+            t, stats = measure(case, 0 if name == "plain" else 1, args.steps, args.warmup)
+        except _lib.IcarAdvError as e:  # e.g. a variant that does not fit in memory
+            log(f"  {name}: not measured ({e})")
+            results[name] = {"error": str(e)}
+            continue
+        lc = stats.pop("_launch_config", None)
+        results[name] = {"seconds": t, "value": cells * S * args.steps / t, "stats": stats,
+                         "b_step": step_bytes(d), "launch_config": lc}
+        log(f"  {name}: {results[name]['value'] / 1e9:.2f} G cell-species/s, "
+            f"{1e3 * t / args.steps:.1f} ms/step")
+    d.storage.clear()
+    d.storage.update(mixed)
+    r = results["batched_mixed"]
+    roof = roofline_of(r["stats"], d, cells * S, "c5_4096x4096x50", "fused_corrective_pass")
+    variants = {}
+    for k, v in results.items():
+        if "error" in v:
+            variants[k] = v
+            continue
+        variants[k] = {"value": v["value"], "ms_per_step": 1e3 * v["seconds"] / args.steps,
+                       "compulsory_hbm_fraction": v["b_step"] * args.steps / v["seconds"] / HBM_PEAK,
+                       "bytes_per_cell_species": v["b_step"] / (cells * S),
+                       "launch_config": v["launch_config"] if k != "plain" else None,
+                       "kernels": kernel_table(v["stats"])}
     out = {
         "metric": "MPDATA cell-species updates/s & % HBM roofline, plain FP32 vs batched/mixed",
-        "value": value, "unit": "cell-species updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * t / args.steps, "higher_is_better": True,
+        "value": r["value"], "unit": "cell-species updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * r["seconds"] / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "storage": {n: ("f32" if n not in d.storage else "bf16") for n in d.species},
-        "config": {"workload": f"single domain {d.nx}x{d.ny}x{d.nz}, {S} species (qv, theta "
-                               "FP32; ten BF16 with power-of-two scaling), mpdata_order 2 + FCT, "
+        "storage": {n: ("f32" if n not in d.storage else _lib.STORE_NAMES[d.storage[n][0]])
+                    for n in d.species},
+        "parity": "batched and plain bit-identical to the FP32 oracle; batched_mixed equal to the "
+                  "FP32 oracle stepped on the 16-bit values with one rounding per step",
+        "config": {"workload": f"single domain {d.nx}x{d.ny}x{d.nz}, {S} species (qv, theta FP32; "
+                               "ten BF16 with power-of-two scaling), mpdata_order 2 + FCT, "
                                "advect_density, Courant 0.5",
                    "nx": d.nx, "ny": d.ny, "nz": d.nz, "species": S, "mpdata_order": 2,
-                   "fct": True, "dt": case.dt, "generation_s": gen_s, "launch_config": lc},
-        "compulsory_hbm_fraction": b_step * args.steps / t / HBM_PEAK,
-        "bytes_per_cell_species": b_step / (cells * S),
-        "kernels": {k: {"avg_ms": v["total_ms"] / v["launches"], "launches": v["launches"],
-                        "achieved_GBps": v["alg_bytes"] / v["total_ms"] / 1e6}
-                    for k, v in stats.items()},
+                   "fct": True, "variant": "batched_mixed", "dt": case.dt,
+                   "generation_s": gen_s, "variants_run": todo},
+        "roofline": roof,
+        "compulsory_hbm_fraction": r["b_step"] * args.steps / r["seconds"] / HBM_PEAK,
+        "variants": variants,
     }
-    print(json.dumps(out), flush=True)
-    return 0
+    if "value" in results.get("plain", {}):
+        out["mixed_vs_plain"] = r["value"] / results["plain"]["value"]
+    if "value" in results.get("batched", {}):
+        out["mixed_vs_batched_fp32"] = r["value"] / results["batched"]["value"]
+    if not args.no_cpu:
+        log("CPU baseline (oracle, one image per thread on a 512-row strip) ...")
+        out["cpu_baseline"] = cpu_baseline(case, rows=512)
+    return out
 
 
 def main() -> int:
@@ -361,9 +490,16 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=("c5", "c2"), default="c5",
+                    help="c5: the 4096x4096x50 domain, 12 species, mixed precision (the "
+                         "headline); c2: the 1024x1024x30, 10-species domain in the headline line")
     ap.add_argument("--variant", choices=("plain", "batched", "batched_bf16", "all"),
-                    default="all")
-    ap.add_argument("--no-cpu", action="store_true")
+                    default="all", help="variants of the c2 domain")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain variant of the 4096x4096x50 domain")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="with c5: skip the 1024x1024x30 domain and its companions")
     ap.add_argument("--no-small", action="store_true", help="skip the launch-bound small case")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 48-domain ensemble")
     ap.add_argument("--no-std", action="store_true",
@@ -372,20 +508,32 @@ def main() -> int:
                     help="skip the host transfer and snapshot measurement")
     ap.add_argument("--no-wind", action="store_true",
                     help="skip the wind-balance, dt and forcing kernels' timing")
-    ap.add_argument("--workload", choices=("c2", "c5"), default="c2",
-                    help="c2: the 1024x1024x30 single domain (the headline line); c5: the "
-                         "4096x4096x50 mixed-precision domain alone")
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--ny", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU decomposition is a design, not part of this build)",
-              file=sys.stderr)
+              "(every GPU box of this build has one GPU; the multi-GPU decomposition is a "
+              "design, not part of this build)", file=sys.stderr)
         return 2
+    if args.workload == "c2":
+        out = c2_line(args, cpu=not args.no_cpu)
+    else:
+        out = c5_line(args)
+        if not args.no_extras:
+            log("the 1024x1024x30, 10-species domain and its companions ...")
+            c2 = c2_line(args, cpu=False)
+            for k in ("metric", "unit", "n_gpus", "steps", "warmup", "higher_is_better",
+                      "scaling", "vs_baseline", "data"):
+                c2.pop(k, None)
+            out["c2_1024x1024x30"] = c2
+    print(json.dumps(out), flush=True)
+    return 0
 
-    if args.workload == "c5":
-        return bench_c5(args)
+
+def c2_line(args, cpu: bool) -> dict:
+    """The 1024x1024x30, 10-species domain at mpdata_order 3 + FCT in its
+    variants (the best bit-exact one as the value), and its companions."""
     from hicar_amd import synthetic
     log("generating the 1024x1024x30, 10-species workload ...")
     case = synthetic.config2(nx=args.nx, ny=args.ny)
@@ -504,11 +652,10 @@ def main() -> int:
         "host_handoff": handoff,
         "wind_steps": wind,
     }
-    if not args.no_cpu:
-        log("CPU baseline (oracle: one thread per species, then one thread) ...")
+    if cpu:
+        log("CPU baseline (oracle, one image per thread) ...")
         out["cpu_baseline"] = cpu_baseline(case)
-    print(json.dumps(out), flush=True)
-    return 0
+    return out
 
 
 if __name__ == "__main__":

@@ -111,6 +111,11 @@ class Advection:
             self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
+    def set_compensation(self, on: bool) -> None:
+        """The fused kernels' flux divergence with compensated sums (not the
+        reference's operation order) or, by default, without."""
+        self._check(self.lib.icar_adv_set_compensation(self.h, int(bool(on))))
+
     def divergence(self) -> tuple[int, float]:
         """test_divergence (adv_mpdata.f90:445-475) on the Courant numbers of
         the last step: the number of interior cells with |div| > 1e-3 and the

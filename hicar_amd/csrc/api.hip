@@ -160,6 +160,7 @@ struct Handle {
     // counts species bindings: each may reallocate device buffers and the
     // fused path's tables, which recorded graphs and ensembles point at
     unsigned bind_gen = 0;
+    int compensated = 0;  // icar_adv_set_compensation
     // graph capture of icar_adv_run
     bool use_graph = false;
     hipGraphExec_t gexec = nullptr;
@@ -436,7 +437,8 @@ int advect_species_exact(Handle *h, float *q)
 int ensure_batch(Handle *h)
 {
     if (h->batch) return 0;
-    FastDomain fd{h->fast, domain_fields(h), h->sf, h->g.dx, h->o.advect_density};
+    FastDomain fd{h->fast, domain_fields(h), h->sf, h->g.dx, h->o.advect_density,
+                  h->compensated};
     std::string e;
     h->batch = fast_batch_create(&fd, 1, h->o.mpdata_order, h->o.mpdata_fct, &e);
     if (!h->batch) return set_err(h, ICAR_ADV_EINVAL, "%s", e.c_str());
@@ -1279,6 +1281,29 @@ int icar_adv_max_courant(void *handle, double out[3])
     return 0;
 }
 
+int icar_adv_set_compensation(void *handle, int on)
+{
+    Handle *h;
+    int rc = check_handle(handle, &h);
+    if (rc) return rc;
+    if (!h->fast)
+        return set_err(h, ICAR_ADV_EINVAL,
+                       "compensated sums are an option of the fused MPDATA kernels "
+                       "(kernel_variant 1)");
+    h->compensated = on ? 1 : 0;
+    // the batch's tables and a recorded run hold the old setting; ensembles
+    // built over this handle refuse to run, as after a binding
+    h->bind_gen++;
+    if (h->gexec) {
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipGraphExecDestroy(h->gexec);
+        h->gexec = nullptr;
+    }
+    fast_batch_destroy(h->batch);
+    h->batch = nullptr;
+    return 0;
+}
+
 int icar_adv_divergence(void *handle, double out[2])
 {
     Handle *h;
@@ -2032,7 +2057,8 @@ int icar_adv_ensemble_create(int n, void *const *handles, void **ens)
         (void)hipStreamSynchronize(h->stream);
         e->hs.push_back(h);
         e->gens.push_back(h->bind_gen);
-        doms.push_back({h->fast, domain_fields(h), h->sf, h->g.dx, h->o.advect_density});
+        doms.push_back({h->fast, domain_fields(h), h->sf, h->g.dx, h->o.advect_density,
+                        h->compensated});
     }
     std::string m;
     e->batch = fast_batch_create(doms.data(), n, e->hs[0]->o.mpdata_order, e->hs[0]->o.mpdata_fct, &m);
@@ -2164,6 +2190,12 @@ int icar_adv_ensemble_kernel_stats(void *ens, int id, double *total_ms, long lon
 }  // extern "C"
 
 namespace icaradv {
+int set_call_error(int code, const char *msg)
+{
+    g_create_error = msg;
+    return code;
+}
+
 const char *kernel_name(int id)
 {
     static const char *names[K_COUNT] = {"courant",     "upwind",     "pseudo_velocity",

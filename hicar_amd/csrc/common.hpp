@@ -62,6 +62,10 @@ enum KernelId {
 
 const char *kernel_name(int id);
 
+// Records the message of a failed call that has no handle (what
+// icar_adv_last_error(NULL) returns) and returns code.
+int set_call_error(int code, const char *msg);
+
 // Per-step derived fields written by the Courant kernel (species invariant).
 struct StepFields {
     float *U, *V, *W;  // Courant numbers (mpdata_compute_wind)

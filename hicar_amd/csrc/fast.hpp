@@ -39,6 +39,7 @@ struct FastDomain {
     StepFields sf;
     float dx;
     int advect_density;
+    int compensated;  // the flux divergence with compensated sums (icar_adv_set_compensation)
 };
 struct FastBatch;
 FastBatch *fast_batch_create(const FastDomain *doms, int n, int order, int fct, std::string *err);

@@ -38,8 +38,11 @@
 #include <string>
 #include <vector>
 
+#include <type_traits>
+
 #include "../../include/icar_adv.h"
 #include "fast.hpp"
+#include "fdiv.hpp"
 
 namespace icaradv {
 
@@ -198,6 +201,7 @@ struct DomDesc {
     int nks, kch;         // corrective kernel: column chunks along k and levels per chunk
     float dx, dt;
     int advect_density;
+    int compensated;      // flux divergence with compensated sums
     DomainFields f;       // raw inputs
     StepFields sf;        // Courant numbers and products (written by k_courant_b)
     float *coef;          // face coefficients (float4 groups)
@@ -482,10 +486,76 @@ constexpr int SR = 3;             // staged-row slots per wave (the third only f
 constexpr int OWN = 2;            // output rows per wave: ly + 8*(rb + t), t = 0, 1
 
 
+// s = RN(a + b) and its rounding error e, a + b = s + e exactly (Knuth's
+// TwoSum; the unit is compiled without contraction)
+__device__ __forceinline__ float two_sum(float a, float b, float &e)
+{
+    const float s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+    return s;
+}
+
+// The opt-in compensated form of the update q1 - ((fr-fl) + (fn-fs))/G -
+// (Fzt-Fzb)/Gz (upwind_advection, adv_mpdata.f90:86-97): the flux sums and
+// the two subtractions carry their rounding errors (TwoSum), the quotients
+// their remainders (one FMA each), and the errors enter once, before the
+// final rounding; the result is q1 minus the exact divergence of the FP32
+// fluxes, rounded about once.  Not the reference's operation order.
+__device__ __forceinline__ float divergence_compensated(float q1, float fl, float fr, float fs,
+                                                        float fn, float fzt, float fzb, float G,
+                                                        float Gz)
+{
+    float ea, eb, eh, ez, e1, e2;
+    const float a = two_sum(fr, -fl, ea);
+    const float b = two_sum(fn, -fs, eb);
+    const float h = two_sum(a, b, eh);
+    const float dh = h / G;
+    const float ch = (__builtin_fmaf(-dh, G, h) + ((ea + eb) + eh)) / G;
+    const float vz = two_sum(fzt, -fzb, ez);
+    const float dz = vz / Gz;
+    const float cz = (__builtin_fmaf(-dz, Gz, vz) + ez) / Gz;
+    const float t = two_sum(q1, -dh, e1);
+    const float u = two_sum(t, -dz, e2);
+    return u + ((e1 + e2) - (ch + cz));
+}
+
+// x/d: the IEEE division, or (F) the three-operation quotient of fdiv.hpp
+// with the operands added to the guard g
+template <bool F>
+__device__ __forceinline__ float fdiv(float x, float d, DivGuard &g)
+{
+    if constexpr (F) {
+        guard_div(g, x, d);
+        return div_y(x, d, recip_nr(d));
+    } else {
+        return x / d;
+    }
+}
+
+// the same with y = recip_of(d, g) computed once for several numerators
+template <bool F>
+__device__ __forceinline__ float fdiv_y(float x, float d, float y, DivGuard &g)
+{
+    if constexpr (F) {
+        guard_num(g, x);
+        return div_y(x, d, y);
+    } else {
+        return x / d;
+    }
+}
+
+__device__ __forceinline__ float recip_of(float d, DivGuard &g)
+{
+    guard_den(g, d);
+    return recip_nr(d);
+}
+
 // WPE: waves per SIMD the register allocation targets (2 blocks of 8 waves
 // per CU need 4); PF: load this iteration's coefficients before the first
-// barrier (latency hidden by the barrier) or at their use.
-template <bool FCT, int WPE, bool PF, bool Q16>
+// barrier (latency hidden by the barrier) or at their use; FD: the
+// three-operation quotient where its operands are in range (fdiv.hpp).
+template <bool FCT, int WPE, bool PF, bool Q16, bool FD>
 __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
     k_corr(const DomDesc *__restrict__ DD, int nd, int pass, unsigned nwork)
 {
@@ -667,94 +737,127 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
         __syncthreads();
         if (!PF) load_coef();
 
+        // Phases B, C and D divide as the reference does, one IEEE quotient
+        // per division.  With FD each quotient is the three-operation one of
+        // fdiv.hpp, which is the same value wherever its operands are in
+        // range; a guard collects the operands of a wave's quotients, and a
+        // wave with an operand out of range computes the phase again with
+        // the IEEE division before anything is written.
+        using fast_t = std::integral_constant<bool, true>;
+        using ieee_t = std::integral_constant<bool, false>;
+
         // ---------------- (B) unlimited pseudo-velocities (mpdata_fluxes)
-        // z face on top of level L1 for the owned columns (:211-246, :380)
+        // (the x and y faces go to LDS as they are computed: a phase computed
+        // again writes the same places; the z faces are kept until the phase
+        // is accepted, since they shift the column's pipeline)
+        float zw[OWN], zf[OWN];
+        auto phase_b = [&](auto fd_c) -> bool {
+            constexpr bool F = decltype(fd_c)::value;
+            DivGuard g;
+            auto dv = [&](float x, float d) { return fdiv<F>(x, d, g); };
+            // z face on top of level L1 for the owned columns (:211-246, :380)
+            if (L1 >= 0 && L1 <= nz - 1) {
+                const bool kz_ok = L1 <= nz - 2;
+#pragma unroll
+                for (int t = 0; t < OWN; t++) {
+                    const int row = ly + 8 * (rb + t);
+                    const int gj = j0 + row;
+                    float wv = 0.0f, fv = 0.0f;
+                    if (kz_ok && lane_out && own_in[t]) {
+                        const float(*qa)[CX] = sq1[L1 & 3];  // level k
+                        const float(*qb)[CX] = sq1[m & 3];   // level k+1
+                        const float l = qa[row][lx], r = qb[row][lx];
+                        float val = dv(kz[t][0] * (r - l), r + l + eps);
+                        const bool cu = gi >= 1 && gi <= nx - 2, cv = gj >= 1 && gj <= ny - 2;
+                        const float ys = (F && (cu || cv)) ? recip_of(kz[t][3], g) : 0.0f;
+                        if (cu) {  // WxU (:227-233)
+                            const float eq = dv(qb[row][lx + 1] - qa[row][lx - 1] + qa[row][lx + 1] - qb[row][lx - 1],
+                                                qa[row][lx + 1] + qb[row][lx + 1] + qa[row][lx - 1] + qb[row][lx - 1] + eps);
+                            val = val - fdiv_y<F>(kz[t][1] * eq, kz[t][3], ys, g);
+                        }
+                        if (cv) {  // WxV (:238-245)
+                            const float eq = dv(qb[row + 1][lx] - qa[row - 1][lx] + qa[row + 1][lx] - qb[row - 1][lx],
+                                                qa[row + 1][lx] + qb[row - 1][lx] + qb[row + 1][lx] + qa[row - 1][lx] + eps);
+                            val = val - fdiv_y<F>(kz[t][2] * eq, kz[t][3], ys, g);
+                        }
+                        wv = val * 0.5f * kz[t][4];
+                        fv = flux1(l, r, wv);
+                    }
+                    zw[t] = wv;
+                    zf[t] = fv;
+                }
+            }
+            if (L2 >= 0) {
+                const float(*q1m)[CX] = sq1[L2 & 3];
+                const float(*q1u)[CX] = sq1[(L2 + 1) & 3];
+                const float(*q1d)[CX] = sq1[(L2 + 3) & 3];
+                const bool kin = L2 >= 1 && L2 <= nz - 2;
+                // x faces (left of column gi) on the owned rows, lanes 1..63 (:131-165, :378)
+#pragma unroll
+                for (int t = 0; t < OWN; t++) {
+                    const int row = ly + 8 * (rb + t);
+                    const int gj = j0 + row;
+                    float uv = 0.0f, fv = 0.0f;
+                    if (lx >= 1 && gi >= 1 && own_in[t]) {
+                        const float r = q1m[row][lx], l = q1m[row][lx - 1];
+                        float val = dv(kx[t][0] * (r - l), r + l + eps);
+                        const bool cv = gj >= 1 && gj <= ny - 2;
+                        const float ys = (F && (cv || kin)) ? recip_of(kx[t][3], g) : 0.0f;
+                        if (cv) {  // UxV (:145-152)
+                            const float eq = dv(q1m[row + 1][lx] - q1m[row - 1][lx] + q1m[row + 1][lx - 1] - q1m[row - 1][lx - 1],
+                                                q1m[row + 1][lx] + q1m[row - 1][lx] + q1m[row + 1][lx - 1] + q1m[row - 1][lx - 1] + eps);
+                            val = val - fdiv_y<F>(kx[t][1] * eq, kx[t][3], ys, g);
+                        }
+                        if (kin) {  // UxW (:157-164)
+                            const float eq = dv(q1u[row][lx] - q1d[row][lx] + q1u[row][lx - 1] - q1d[row][lx - 1],
+                                                q1u[row][lx] + q1d[row][lx] + q1u[row][lx - 1] + q1d[row][lx - 1] + eps);
+                            val = val - fdiv_y<F>(kx[t][2] * eq, kx[t][3], ys, g);
+                        }
+                        uv = val * 0.5f;
+                        fv = flux1(l, r, uv);
+                    }
+                    sxu[row][lx] = uv;
+                    sxf[row][lx] = fv;
+                }
+                // y faces (south of row gj) on the staged rows 1..19, lanes 2..61 (:169-205, :379)
+#pragma unroll
+                for (int r4 = 0; r4 < SR; r4++) {
+                    const int row = ly + 8 * r4;
+                    if (r4 >= nsr || row < 1) continue;
+                    const int gj = j0 + row;
+                    float vv = 0.0f, fv = 0.0f;
+                    if (lane_out && gj >= 1 && gj <= ny - 1) {
+                        const float r = q1m[row][lx], l = q1m[row - 1][lx];
+                        float val = dv(ky[r4][0] * (r - l), r + l + eps);
+                        const bool cu = gi >= 1 && gi <= nx - 2;
+                        const float ys = (F && (cu || kin)) ? recip_of(ky[r4][3], g) : 0.0f;
+                        if (cu) {  // VxU (:186-192)
+                            const float eq = dv(q1m[row - 1][lx + 1] - q1m[row][lx - 1] + q1m[row][lx + 1] - q1m[row - 1][lx - 1],
+                                                q1m[row][lx + 1] + q1m[row - 1][lx + 1] + q1m[row][lx - 1] + q1m[row - 1][lx - 1] + eps);
+                            val = val - fdiv_y<F>(ky[r4][1] * eq, ky[r4][3], ys, g);
+                        }
+                        if (kin) {  // VxW (:197-204)
+                            const float eq = dv(q1u[row - 1][lx] - q1d[row][lx] + q1u[row][lx] - q1d[row - 1][lx],
+                                                q1u[row - 1][lx] + q1d[row][lx] + q1u[row][lx] + q1d[row - 1][lx] + eps);
+                            val = val - fdiv_y<F>(ky[r4][2] * eq, ky[r4][3], ys, g);
+                        }
+                        vv = val * 0.5f;
+                        fv = flux1(l, r, vv);
+                    }
+                    syu[row][lx] = vv;
+                    syf[row][lx] = fv;
+                }
+            }
+            return guard_ok(g);
+        };
+        if (!FD || __any(!phase_b(fast_t{}))) phase_b(ieee_t{});
         if (L1 >= 0 && L1 <= nz - 1) {
-            const bool kz_ok = L1 <= nz - 2;
 #pragma unroll
             for (int t = 0; t < OWN; t++) {
-                const int row = ly + 8 * (rb + t);
-                const int gj = j0 + row;
-                float wv = 0.0f, fv = 0.0f;
-                if (kz_ok && lane_out && own_in[t]) {
-                    const float(*qa)[CX] = sq1[L1 & 3];  // level k
-                    const float(*qb)[CX] = sq1[m & 3];   // level k+1
-                    const float l = qa[row][lx], r = qb[row][lx];
-                    float val = kz[t][0] * (r - l) / (r + l + eps);
-                    if (gi >= 1 && gi <= nx - 2) {  // WxU (:227-233)
-                        const float eq = (qb[row][lx + 1] - qa[row][lx - 1] + qa[row][lx + 1] - qb[row][lx - 1]) /
-                                         (qa[row][lx + 1] + qb[row][lx + 1] + qa[row][lx - 1] + qb[row][lx - 1] + eps);
-                        val = val - kz[t][1] * eq / kz[t][3];
-                    }
-                    if (gj >= 1 && gj <= ny - 2) {  // WxV (:238-245)
-                        const float eq = (qb[row + 1][lx] - qa[row - 1][lx] + qa[row + 1][lx] - qb[row - 1][lx]) /
-                                         (qa[row + 1][lx] + qb[row - 1][lx] + qb[row + 1][lx] + qa[row - 1][lx] + eps);
-                        val = val - kz[t][2] * eq / kz[t][3];
-                    }
-                    wv = val * 0.5f * kz[t][4];
-                    fv = flux1(l, r, wv);
-                }
                 wu2[t] = wu1[t];
                 fz2[t] = fz1[t];
-                wu1[t] = wv;
-                fz1[t] = fv;
-            }
-        }
-        if (L2 >= 0) {
-            const float(*q1m)[CX] = sq1[L2 & 3];
-            const float(*q1u)[CX] = sq1[(L2 + 1) & 3];
-            const float(*q1d)[CX] = sq1[(L2 + 3) & 3];
-            const bool kin = L2 >= 1 && L2 <= nz - 2;
-            // x faces (left of column gi) on the owned rows, lanes 1..63 (:131-165, :378)
-#pragma unroll
-            for (int t = 0; t < OWN; t++) {
-                const int row = ly + 8 * (rb + t);
-                const int gj = j0 + row;
-                float uv = 0.0f, fv = 0.0f;
-                if (lx >= 1 && gi >= 1 && own_in[t]) {
-                    const float r = q1m[row][lx], l = q1m[row][lx - 1];
-                    float val = kx[t][0] * (r - l) / (r + l + eps);
-                    if (gj >= 1 && gj <= ny - 2) {  // UxV (:145-152)
-                        const float eq = (q1m[row + 1][lx] - q1m[row - 1][lx] + q1m[row + 1][lx - 1] - q1m[row - 1][lx - 1]) /
-                                         (q1m[row + 1][lx] + q1m[row - 1][lx] + q1m[row + 1][lx - 1] + q1m[row - 1][lx - 1] + eps);
-                        val = val - kx[t][1] * eq / kx[t][3];
-                    }
-                    if (kin) {  // UxW (:157-164)
-                        const float eq = (q1u[row][lx] - q1d[row][lx] + q1u[row][lx - 1] - q1d[row][lx - 1]) /
-                                         (q1u[row][lx] + q1d[row][lx] + q1u[row][lx - 1] + q1d[row][lx - 1] + eps);
-                        val = val - kx[t][2] * eq / kx[t][3];
-                    }
-                    uv = val * 0.5f;
-                    fv = flux1(l, r, uv);
-                }
-                sxu[row][lx] = uv;
-                sxf[row][lx] = fv;
-            }
-            // y faces (south of row gj) on the staged rows 1..19, lanes 2..61 (:169-205, :379)
-#pragma unroll
-            for (int r4 = 0; r4 < SR; r4++) {
-                const int row = ly + 8 * r4;
-                if (r4 >= nsr || row < 1) continue;
-                const int gj = j0 + row;
-                float vv = 0.0f, fv = 0.0f;
-                if (lane_out && gj >= 1 && gj <= ny - 1) {
-                    const float r = q1m[row][lx], l = q1m[row - 1][lx];
-                    float val = ky[r4][0] * (r - l) / (r + l + eps);
-                    if (gi >= 1 && gi <= nx - 2) {  // VxU (:186-192)
-                        const float eq = (q1m[row - 1][lx + 1] - q1m[row][lx - 1] + q1m[row][lx + 1] - q1m[row - 1][lx - 1]) /
-                                         (q1m[row][lx + 1] + q1m[row - 1][lx + 1] + q1m[row][lx - 1] + q1m[row - 1][lx - 1] + eps);
-                        val = val - ky[r4][1] * eq / ky[r4][3];
-                    }
-                    if (kin) {  // VxW (:197-204)
-                        const float eq = (q1u[row - 1][lx] - q1d[row][lx] + q1u[row][lx] - q1d[row - 1][lx]) /
-                                         (q1u[row - 1][lx] + q1d[row][lx] + q1u[row][lx] + q1d[row - 1][lx] + eps);
-                        val = val - ky[r4][2] * eq / ky[r4][3];
-                    }
-                    vv = val * 0.5f;
-                    fv = flux1(l, r, vv);
-                }
-                syu[row][lx] = vv;
-                syf[row][lx] = fv;
+                wu1[t] = zw[t];
+                fz1[t] = zf[t];
             }
         }
         __syncthreads();
@@ -764,103 +867,120 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
 #pragma unroll
         for (int t = 0; t < OWN; t++) Fzc[t] = 0.0f;
         if (FCT) {
-            if (L2 >= 0) {
-                const float(*q1m)[CX] = sq1[L2 & 3];
-                const float(*q0m)[CX] = sq0[L2 % 3];
-                // x cells of the owned rows, lanes 1..62
+            float zbi[OWN], zbo[OWN], fzc[OWN];  // the x and y betas go to LDS directly
+            auto phase_c = [&](auto fd_c) -> bool {
+                constexpr bool F = decltype(fd_c)::value;
+                DivGuard g;
+                auto dv = [&](float x, float d) { return fdiv<F>(x, d, g); };
+                if (L2 >= 0) {
+                    const float(*q1m)[CX] = sq1[L2 & 3];
+                    const float(*q0m)[CX] = sq0[L2 % 3];
+                    // x cells of the owned rows, lanes 1..62
 #pragma unroll
-                for (int t = 0; t < OWN; t++) {
-                    const int row = ly + 8 * (rb + t);
-                    if (lx >= 1 && lx <= CX - 2 && lane_in) {
-                        float qmax, qmin, fin = 0.0f, fout = 0.0f;
-                        const float qc1 = q1m[row][lx];
-                        if (gi == 0) {
-                            qmax = fmaxf(fmaxf(qc1, q1m[row][lx + 1]), fmaxf(q0m[row][lx], q0m[row][lx + 1]));
-                            qmin = fminf(fminf(qc1, q1m[row][lx + 1]), fminf(q0m[row][lx], q0m[row][lx + 1]));
-                        } else if (gi == nx - 1) {
-                            qmax = max3(q1m[row][lx - 1], qc1, q0m[row][lx - 1]);
-                            qmin = min3(q1m[row][lx - 1], qc1, q0m[row][lx - 1]);
-                        } else {
-                            qmax = fmaxf(max3(q1m[row][lx - 1], qc1, q1m[row][lx + 1]),
-                                         max3(q0m[row][lx - 1], q0m[row][lx], q0m[row][lx + 1]));
-                            qmin = fminf(min3(q1m[row][lx - 1], qc1, q1m[row][lx + 1]),
-                                         min3(q0m[row][lx - 1], q0m[row][lx], q0m[row][lx + 1]));
-                            const float fl = sxf[row][lx], fr = sxf[row][lx + 1];
-                            fin = fmaxf(0.0f, fl) - fminf(0.0f, fr);
-                            fout = fmaxf(0.0f, fr) - fminf(0.0f, fl);
+                    for (int t = 0; t < OWN; t++) {
+                        const int row = ly + 8 * (rb + t);
+                        if (lx >= 1 && lx <= CX - 2 && lane_in) {
+                            float qmax, qmin, fin = 0.0f, fout = 0.0f;
+                            const float qc1 = q1m[row][lx];
+                            if (gi == 0) {
+                                qmax = fmaxf(fmaxf(qc1, q1m[row][lx + 1]), fmaxf(q0m[row][lx], q0m[row][lx + 1]));
+                                qmin = fminf(fminf(qc1, q1m[row][lx + 1]), fminf(q0m[row][lx], q0m[row][lx + 1]));
+                            } else if (gi == nx - 1) {
+                                qmax = max3(q1m[row][lx - 1], qc1, q0m[row][lx - 1]);
+                                qmin = min3(q1m[row][lx - 1], qc1, q0m[row][lx - 1]);
+                            } else {
+                                qmax = fmaxf(max3(q1m[row][lx - 1], qc1, q1m[row][lx + 1]),
+                                             max3(q0m[row][lx - 1], q0m[row][lx], q0m[row][lx + 1]));
+                                qmin = fminf(min3(q1m[row][lx - 1], qc1, q1m[row][lx + 1]),
+                                             min3(q0m[row][lx - 1], q0m[row][lx], q0m[row][lx + 1]));
+                                const float fl = sxf[row][lx], fr = sxf[row][lx + 1];
+                                fin = fmaxf(0.0f, fl) - fminf(0.0f, fr);
+                                fout = fmaxf(0.0f, fr) - fminf(0.0f, fl);
+                            }
+                            sbxi[row][lx] = dv(qmax - qc1, fin + eps15);
+                            sbxo[row][lx] = dv(qc1 - qmin, fout + eps15);
                         }
-                        sbxi[row][lx] = (qmax - qc1) / (fin + eps15);
-                        sbxo[row][lx] = (qc1 - qmin) / (fout + eps15);
+                    }
+                    // y cells on the staged rows 1..18, lanes 2..61
+#pragma unroll
+                    for (int r4 = 0; r4 < SR; r4++) {
+                        const int row = ly + 8 * r4;
+                        if (r4 >= nsr || row < 1 || row > CY - 2) continue;
+                        const int gj = j0 + row;
+                        if (lane_out && gj >= 0 && gj < ny) {
+                            float qmax, qmin, fin = 0.0f, fout = 0.0f;
+                            const float qc1 = q1m[row][lx];
+                            if (gj == 0) {
+                                qmax = fmaxf(fmaxf(qc1, q1m[row + 1][lx]), fmaxf(q0m[row][lx], q0m[row + 1][lx]));
+                                qmin = fminf(fminf(qc1, q1m[row + 1][lx]), fminf(q0m[row][lx], q0m[row + 1][lx]));
+                            } else if (gj == ny - 1) {
+                                qmax = max3(q1m[row - 1][lx], qc1, q0m[row - 1][lx]);
+                                qmin = min3(q1m[row - 1][lx], qc1, q0m[row - 1][lx]);
+                            } else {
+                                qmax = fmaxf(max3(q1m[row - 1][lx], qc1, q1m[row + 1][lx]),
+                                             max3(q0m[row - 1][lx], q0m[row][lx], q0m[row + 1][lx]));
+                                qmin = fminf(min3(q1m[row - 1][lx], qc1, q1m[row + 1][lx]),
+                                             min3(q0m[row - 1][lx], q0m[row][lx], q0m[row + 1][lx]));
+                                const float fs = syf[row][lx], fn = syf[row + 1][lx];
+                                fin = fmaxf(0.0f, fs) - fminf(0.0f, fn);
+                                fout = fmaxf(0.0f, fn) - fminf(0.0f, fs);
+                            }
+                            sbyi[row][lx] = dv(qmax - qc1, fin + eps15);
+                            sbyo[row][lx] = dv(qc1 - qmin, fout + eps15);
+                        }
                     }
                 }
-                // y cells on the staged rows 1..18, lanes 2..61
+                // z: betas of cell L1 of each owned column, then the limited
+                // velocity and flux on the face between L2 and L1 (flux_is_w ends)
+                if (L1 >= 0 && L1 <= nz - 1) {
 #pragma unroll
-                for (int r4 = 0; r4 < SR; r4++) {
-                    const int row = ly + 8 * r4;
-                    if (r4 >= nsr || row < 1 || row > CY - 2) continue;
-                    const int gj = j0 + row;
-                    if (lane_out && gj >= 0 && gj < ny) {
-                        float qmax, qmin, fin = 0.0f, fout = 0.0f;
-                        const float qc1 = q1m[row][lx];
-                        if (gj == 0) {
-                            qmax = fmaxf(fmaxf(qc1, q1m[row + 1][lx]), fmaxf(q0m[row][lx], q0m[row + 1][lx]));
-                            qmin = fminf(fminf(qc1, q1m[row + 1][lx]), fminf(q0m[row][lx], q0m[row + 1][lx]));
-                        } else if (gj == ny - 1) {
-                            qmax = max3(q1m[row - 1][lx], qc1, q0m[row - 1][lx]);
-                            qmin = min3(q1m[row - 1][lx], qc1, q0m[row - 1][lx]);
+                    for (int t = 0; t < OWN; t++) {
+                        const int row = ly + 8 * (rb + t);
+                        const float qc1 = sq1[L1 & 3][row][lx];
+                        float qmax, qmin, fin, fout;
+                        if (L1 == 0) {
+                            const float qu1 = sq1[m & 3][row][lx];
+                            qmax = fmaxf(fmaxf(qc1, qu1), fmaxf(z0b[t], z0c[t]));
+                            qmin = fminf(fminf(qc1, qu1), fminf(z0b[t], z0c[t]));
+                            fin = 0.0f - fminf(0.0f, fz1[t]);
+                            fout = fmaxf(0.0f, fz1[t]);
+                        } else if (L1 == nz - 1) {
+                            const float qd1 = sq1[L2 & 3][row][lx];
+                            qmax = max3(qd1, qc1, z0a[t]);
+                            qmin = min3(qd1, qc1, z0a[t]);
+                            fin = fmaxf(0.0f, fz2[t]) - fminf(0.0f, fz2[t]);
+                            fout = fmaxf(0.0f, fz2[t]) - fminf(0.0f, fz2[t]);
                         } else {
-                            qmax = fmaxf(max3(q1m[row - 1][lx], qc1, q1m[row + 1][lx]),
-                                         max3(q0m[row - 1][lx], q0m[row][lx], q0m[row + 1][lx]));
-                            qmin = fminf(min3(q1m[row - 1][lx], qc1, q1m[row + 1][lx]),
-                                         min3(q0m[row - 1][lx], q0m[row][lx], q0m[row + 1][lx]));
-                            const float fs = syf[row][lx], fn = syf[row + 1][lx];
-                            fin = fmaxf(0.0f, fs) - fminf(0.0f, fn);
-                            fout = fmaxf(0.0f, fn) - fminf(0.0f, fs);
+                            const float qd1 = sq1[L2 & 3][row][lx], qu1 = sq1[m & 3][row][lx];
+                            qmax = fmaxf(max3(qd1, qc1, qu1), max3(z0a[t], z0b[t], z0c[t]));
+                            qmin = fminf(min3(qd1, qc1, qu1), min3(z0a[t], z0b[t], z0c[t]));
+                            fin = fmaxf(0.0f, fz2[t]) - fminf(0.0f, fz1[t]);
+                            fout = fmaxf(0.0f, fz1[t]) - fminf(0.0f, fz2[t]);
                         }
-                        sbyi[row][lx] = (qmax - qc1) / (fin + eps15);
-                        sbyo[row][lx] = (qc1 - qmin) / (fout + eps15);
+                        const float bi = dv(qmax - qc1, fin + eps15);
+                        const float bo = dv(qc1 - qmin, fout + eps15);
+                        fzc[t] = 0.0f;
+                        if (L2 >= 0 && L2 <= nz - 2) {
+                            float wl = wu2[t];
+                            if (wl > 0.0f)
+                                wl = fminf(fminf(1.0f, bi), bzo[t]) * wl;
+                            else if (wl < 0.0f)
+                                wl = fminf(fminf(1.0f, bzi[t]), bo) * wl;
+                            fzc[t] = flux1(sq1[L2 & 3][row][lx], qc1, wl);
+                        }
+                        zbi[t] = bi;
+                        zbo[t] = bo;
                     }
                 }
-            }
-            // z: betas of cell L1 of each owned column, then the limited
-            // velocity and flux on the face between L2 and L1 (flux_is_w ends)
+                return guard_ok(g);
+            };
+            if (!FD || __any(!phase_c(fast_t{}))) phase_c(ieee_t{});
             if (L1 >= 0 && L1 <= nz - 1) {
 #pragma unroll
                 for (int t = 0; t < OWN; t++) {
-                    const int row = ly + 8 * (rb + t);
-                    const float qc1 = sq1[L1 & 3][row][lx];
-                    float qmax, qmin, fin, fout;
-                    if (L1 == 0) {
-                        const float qu1 = sq1[m & 3][row][lx];
-                        qmax = fmaxf(fmaxf(qc1, qu1), fmaxf(z0b[t], z0c[t]));
-                        qmin = fminf(fminf(qc1, qu1), fminf(z0b[t], z0c[t]));
-                        fin = 0.0f - fminf(0.0f, fz1[t]);
-                        fout = fmaxf(0.0f, fz1[t]);
-                    } else if (L1 == nz - 1) {
-                        const float qd1 = sq1[L2 & 3][row][lx];
-                        qmax = max3(qd1, qc1, z0a[t]);
-                        qmin = min3(qd1, qc1, z0a[t]);
-                        fin = fmaxf(0.0f, fz2[t]) - fminf(0.0f, fz2[t]);
-                        fout = fmaxf(0.0f, fz2[t]) - fminf(0.0f, fz2[t]);
-                    } else {
-                        const float qd1 = sq1[L2 & 3][row][lx], qu1 = sq1[m & 3][row][lx];
-                        qmax = fmaxf(max3(qd1, qc1, qu1), max3(z0a[t], z0b[t], z0c[t]));
-                        qmin = fminf(min3(qd1, qc1, qu1), min3(z0a[t], z0b[t], z0c[t]));
-                        fin = fmaxf(0.0f, fz2[t]) - fminf(0.0f, fz1[t]);
-                        fout = fmaxf(0.0f, fz1[t]) - fminf(0.0f, fz2[t]);
-                    }
-                    const float bi = (qmax - qc1) / (fin + eps15);
-                    const float bo = (qc1 - qmin) / (fout + eps15);
-                    if (L2 >= 0 && L2 <= nz - 2) {
-                        float wl = wu2[t];
-                        if (wl > 0.0f)
-                            wl = fminf(fminf(1.0f, bi), bzo[t]) * wl;
-                        else if (wl < 0.0f)
-                            wl = fminf(fminf(1.0f, bzi[t]), bo) * wl;
-                        Fzc[t] = flux1(sq1[L2 & 3][row][lx], qc1, wl);
-                    }
-                    bzi[t] = bi;
-                    bzo[t] = bo;
+                    Fzc[t] = fzc[t];
+                    bzi[t] = zbi[t];
+                    bzo[t] = zbo[t];
                 }
             }
         } else {
@@ -874,51 +994,71 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
         // a face shared by two cells is limited identically by both
         if (L2 >= 0) {
             const float(*q1m)[CX] = sq1[L2 & 3];
+            float qn[OWN];
+            auto phase_d = [&](auto fd_c) -> bool {
+                constexpr bool F = decltype(fd_c)::value;
+                DivGuard g;
+                auto dv = [&](float x, float d) { return fdiv<F>(x, d, g); };
+#pragma unroll
+                for (int t = 0; t < OWN; t++) {
+                    const int row = ly + 8 * (rb + t);
+                    const int gj = j0 + row;
+                    qn[t] = 0.0f;
+                    if (lane_out && own_in[t] && L2 >= k0) {
+                        const float qc1 = q1m[row][lx];
+                        float q = qc1;
+                        if (!(gi == 0 || gi == nx - 1 || gj == 0 || gj == ny - 1)) {
+                            float fl, fr, fs, fn;
+                            if (FCT) {
+                                float u = sxu[row][lx];  // face between lx-1 and lx
+                                if (u > 0.0f)
+                                    u = fminf(fminf(1.0f, sbxi[row][lx]), sbxo[row][lx - 1]) * u;
+                                else if (u < 0.0f)
+                                    u = fminf(fminf(1.0f, sbxi[row][lx - 1]), sbxo[row][lx]) * u;
+                                fl = flux1(q1m[row][lx - 1], qc1, u);
+                                u = sxu[row][lx + 1];  // face between lx and lx+1
+                                if (u > 0.0f)
+                                    u = fminf(fminf(1.0f, sbxi[row][lx + 1]), sbxo[row][lx]) * u;
+                                else if (u < 0.0f)
+                                    u = fminf(fminf(1.0f, sbxi[row][lx]), sbxo[row][lx + 1]) * u;
+                                fr = flux1(qc1, q1m[row][lx + 1], u);
+                                float v = syu[row][lx];  // face between row-1 and row
+                                if (v > 0.0f)
+                                    v = fminf(fminf(1.0f, sbyi[row][lx]), sbyo[row - 1][lx]) * v;
+                                else if (v < 0.0f)
+                                    v = fminf(fminf(1.0f, sbyi[row - 1][lx]), sbyo[row][lx]) * v;
+                                fs = flux1(q1m[row - 1][lx], qc1, v);
+                                v = syu[row + 1][lx];  // face between row and row+1
+                                if (v > 0.0f)
+                                    v = fminf(fminf(1.0f, sbyi[row + 1][lx]), sbyo[row][lx]) * v;
+                                else if (v < 0.0f)
+                                    v = fminf(fminf(1.0f, sbyi[row][lx]), sbyo[row + 1][lx]) * v;
+                                fn = flux1(qc1, q1m[row + 1][lx], v);
+                            } else {
+                                fl = sxf[row][lx];
+                                fr = sxf[row][lx + 1];
+                                fs = syf[row][lx];
+                                fn = syf[row + 1][lx];
+                            }
+                            if (D.compensated)
+                                q = divergence_compensated(qc1, fl, fr, fs, fn, Fzc[t], Fzp[t],
+                                                           kg[t][0], kg[t][1]);
+                            else {
+                                q = q - dv((fr - fl) + (fn - fs), kg[t][0]);
+                                q = q - dv(Fzc[t] - Fzp[t], kg[t][1]);
+                            }
+                        }
+                        qn[t] = q;
+                    }
+                }
+                return guard_ok(g);
+            };
+            if (!FD || __any(!phase_d(fast_t{}))) phase_d(ieee_t{});
 #pragma unroll
             for (int t = 0; t < OWN; t++) {
-                const int row = ly + 8 * (rb + t);
-                const int gj = j0 + row;
                 if (lane_out && own_in[t] && L2 >= k0) {
                     const unsigned c = own_base[t] + sk * (unsigned)L2;
-                    const float qc1 = q1m[row][lx];
-                    float q = qc1;
-                    if (!(gi == 0 || gi == nx - 1 || gj == 0 || gj == ny - 1)) {
-                        float fl, fr, fs, fn;
-                        if (FCT) {
-                            float u = sxu[row][lx];  // face between lx-1 and lx
-                            if (u > 0.0f)
-                                u = fminf(fminf(1.0f, sbxi[row][lx]), sbxo[row][lx - 1]) * u;
-                            else if (u < 0.0f)
-                                u = fminf(fminf(1.0f, sbxi[row][lx - 1]), sbxo[row][lx]) * u;
-                            fl = flux1(q1m[row][lx - 1], qc1, u);
-                            u = sxu[row][lx + 1];  // face between lx and lx+1
-                            if (u > 0.0f)
-                                u = fminf(fminf(1.0f, sbxi[row][lx + 1]), sbxo[row][lx]) * u;
-                            else if (u < 0.0f)
-                                u = fminf(fminf(1.0f, sbxi[row][lx]), sbxo[row][lx + 1]) * u;
-                            fr = flux1(qc1, q1m[row][lx + 1], u);
-                            float v = syu[row][lx];  // face between row-1 and row
-                            if (v > 0.0f)
-                                v = fminf(fminf(1.0f, sbyi[row][lx]), sbyo[row - 1][lx]) * v;
-                            else if (v < 0.0f)
-                                v = fminf(fminf(1.0f, sbyi[row - 1][lx]), sbyo[row][lx]) * v;
-                            fs = flux1(q1m[row - 1][lx], qc1, v);
-                            v = syu[row + 1][lx];  // face between row and row+1
-                            if (v > 0.0f)
-                                v = fminf(fminf(1.0f, sbyi[row + 1][lx]), sbyo[row][lx]) * v;
-                            else if (v < 0.0f)
-                                v = fminf(fminf(1.0f, sbyi[row][lx]), sbyo[row + 1][lx]) * v;
-                            fn = flux1(qc1, q1m[row + 1][lx], v);
-                        } else {
-                            fl = sxf[row][lx];
-                            fr = sxf[row][lx + 1];
-                            fs = syf[row][lx];
-                            fn = syf[row + 1][lx];
-                        }
-                        q = q - ((fr - fl) + (fn - fs)) / kg[t][0];
-                        q = q - (Fzc[t] - Fzp[t]) / kg[t][1];
-                    }
-                    st_q(p.out, c, p.kindo, p.inv_scaleo, q,
+                    st_q(p.out, c, p.kindo, p.inv_scaleo, qn[t],
                          is_sr(p.kindo) ? sr_bits(c, s, tstep) : 0u);
                 }
                 Fzp[t] = Fzc[t];
@@ -1181,28 +1321,50 @@ int upload_tables(FastBatch *b, std::string *err)
     return 0;
 }
 
-template <bool FCT, bool Q16>
+template <bool FCT, bool Q16, bool FD>
 void launch_corr_t(unsigned nb, hipStream_t s, const DomDesc *D, int n, int pass, unsigned nw,
                    int cfg)
 {
     switch (cfg) {
-    case 0: hipLaunchKernelGGL((k_corr<FCT, 2, true, Q16>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-    case 1: hipLaunchKernelGGL((k_corr<FCT, 4, true, Q16>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-    case 2: hipLaunchKernelGGL((k_corr<FCT, 2, false, Q16>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-    default: hipLaunchKernelGGL((k_corr<FCT, 4, false, Q16>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+    case 0: hipLaunchKernelGGL((k_corr<FCT, 2, true, Q16, FD>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+    case 1: hipLaunchKernelGGL((k_corr<FCT, 4, true, Q16, FD>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+    case 2: hipLaunchKernelGGL((k_corr<FCT, 2, false, Q16, FD>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+    default: hipLaunchKernelGGL((k_corr<FCT, 4, false, Q16, FD>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
     }
+}
+
+// ICAR_ADV_IEEE_DIV=1 keeps the IEEE division everywhere in the corrective
+// kernel (the same results; for comparison)
+bool fast_division()
+{
+    static const bool on = [] {
+        const char *e = getenv("ICAR_ADV_IEEE_DIV");
+        return !(e && atoi(e) != 0);
+    }();
+    return on;
 }
 
 void launch_corr(const FastBatch *b, hipStream_t s, const DomDesc *D, int n, int pass, int cfg)
 {
     const unsigned nw = b->total_work;
     const unsigned nb = (nw + 7) / 8 * 8;
+    const bool fd = fast_division();
     if (b->fct) {
-        if (b->q16) launch_corr_t<true, true>(nb, s, D, n, pass, nw, cfg);
-        else launch_corr_t<true, false>(nb, s, D, n, pass, nw, cfg);
+        if (b->q16) {
+            if (fd) launch_corr_t<true, true, true>(nb, s, D, n, pass, nw, cfg);
+            else launch_corr_t<true, true, false>(nb, s, D, n, pass, nw, cfg);
+        } else {
+            if (fd) launch_corr_t<true, false, true>(nb, s, D, n, pass, nw, cfg);
+            else launch_corr_t<true, false, false>(nb, s, D, n, pass, nw, cfg);
+        }
     } else {
-        if (b->q16) launch_corr_t<false, true>(nb, s, D, n, pass, nw, cfg);
-        else launch_corr_t<false, false>(nb, s, D, n, pass, nw, cfg);
+        if (b->q16) {
+            if (fd) launch_corr_t<false, true, true>(nb, s, D, n, pass, nw, cfg);
+            else launch_corr_t<false, true, false>(nb, s, D, n, pass, nw, cfg);
+        } else {
+            if (fd) launch_corr_t<false, false, true>(nb, s, D, n, pass, nw, cfg);
+            else launch_corr_t<false, false, false>(nb, s, D, n, pass, nw, cfg);
+        }
     }
 }
 
@@ -1368,6 +1530,7 @@ FastBatch *fast_batch_create(const FastDomain *doms, int n, int order, int fct, 
         D.dx = doms[k].dx;
         D.dt = 0.0f;
         D.advect_density = doms[k].advect_density;
+        D.compensated = doms[k].compensated;
         D.f = doms[k].f;
         D.sf = doms[k].sf;
         D.coef = st->coef;

@@ -325,6 +325,32 @@ int icar_adv_launch_config(void *handle, int *corr_cfg, int *ksplit);
  * drops the recording; the next run records anew. */
 int icar_adv_set_graph(void *handle, int enable);
 
+/* The flux divergence of the fused MPDATA kernels (kernel_variant 1) with
+ * compensated sums (on != 0) or in the reference's operation order (0, the
+ * default): the flux differences, their sum and the two subtractions of the
+ * update q - (dFx + dFy)/G - dFz/Gz (upwind_advection, adv_mpdata.f90:86-97,
+ * in the corrective passes) carry their rounding errors (TwoSum), the two
+ * quotients their remainders, and the errors are added before the last
+ * rounding.  The result then differs from the FP32 reference in the last
+ * bits; the measured effect on the error against FP64 is stated with the
+ * precision tests.  Drops a recorded run; ensembles over the handle must be
+ * created again (as after icar_adv_bind_species).  ICAR_ADV_EINVAL for the
+ * other kernel variants and the standard scheme. */
+int icar_adv_set_compensation(void *handle, int on);
+
+/* Checks, on the current device, the three-operation quotient the kernels
+ * take in place of an IEEE division where both operands are in range (q0 =
+ * x*y, rn = q0*d - x, q = q0 - rn*y with y the hardware reciprocal of d
+ * refined by one Newton step): for every divisor significand in [d_first,
+ * d_first + d_count) (d in [1, 2); d_first + d_count <= 2^23) and every
+ * numerator significand (x in [1, 2)) it must equal x / d bit for bit, the
+ * reciprocal must scale exactly with d's exponent over [-40, 40], and a
+ * negative numerator must give the negated quotient.  *mismatches gets the
+ * number of failures, *first the first found ((d bits << 32) | x bits, or
+ * all ones).  All 2^23 divisors take about a minute.  Synchronises. */
+int icar_adv_verify_division(unsigned d_first, unsigned d_count, unsigned long long *mismatches,
+                             unsigned long long *first);
+
 /* ---- Ensembles ------------------------------------------------------------
  * No reference counterpart (one ICAR image holds one domain): a set of
  * independent domains of uneven sizes advanced together, one launch per

@@ -41,17 +41,40 @@ def test_committed_profile_figures_are_read():
     corrective kernel's entries exist and are plausible for one launch on
     the 1024x1024x30 workload (more than its 5.03 GB of own bytes; billions
     of wave instructions)."""
-    t = bench.read_traffic("k_corr")
-    v = bench.read_valu("k_corr")
+    t = bench.read_traffic("k_corr", "c2_1024x1024x30")
+    v = bench.read_valu("k_corr", "c2_1024x1024x30")
     assert t is not None and 5.0e9 < t < 2.0e10
     assert v is not None and 1.0e9 < v < 1.0e10
 
 
 def test_cpu_baseline_object():
+    """One image per thread, as ICAR runs one image per core: the threads
+    used are the process's CPU share (at most 16), stated with the host's
+    own counts."""
     case = synthetic.config2(nx=48, ny=40, nz=8, steps=1, nspecies=3)
     out = bench.cpu_baseline(case)
-    assert set(out) >= {"value", "unit", "cores", "kind", "sample", "single_thread"}
+    assert set(out) >= {"value", "unit", "cores", "kind", "sample", "single_thread", "host"}
     assert out["kind"] == "port" and out["unit"] == "cell-species updates/s"
-    assert 1 <= out["cores"] <= 3 and out["value"] > 0
+    share = len(os.sched_getaffinity(0))
+    assert out["cores"] == min(16, share) and out["value"] > 0
+    assert out["host"]["logical_cpus"] == os.cpu_count()
     assert out["single_thread"]["cores"] == 1
     json.dumps(out)
+
+
+def test_cpu_baseline_images_cover_the_strip_and_step_like_the_whole():
+    """The images' owned rows, stepped on their own with their 2-row halos,
+    equal the whole strip's step away from the cuts' influence: an image
+    is a subdomain, as ICAR's are (its ring, the halo, is not updated)."""
+    import numpy as np
+    from oracle.oracle import Oracle
+    case = synthetic.config2(nx=40, ny=36, nz=6, steps=1, nspecies=2)
+    d = case.domain
+    orc = Oracle(np.float32)
+    whole = orc.advect(d, case.options, case.dt)
+    sub, sp = bench._image(d, 10, 24)
+    part = orc.advect(sub, case.options, case.dt, species=sp)
+    for n in whole:
+        # rows 11..22 are updated inside the image from the same inputs; the
+        # MPDATA stencil of the corrective pass reaches 2 rows, the limiter 1
+        np.testing.assert_array_equal(part[n][3:-3], whole[n][13:21], err_msg=n)

@@ -176,3 +176,25 @@ def test_autotuned_small_domain_matches_oracle(oracle32, launch_env):
     assert 0 <= cfg <= 3 and 1 <= ks <= 5
     print(f"\n[autotune] 256x64x20, 1 species: corr_cfg {cfg}, column chunks {ks}")
     assert_same({"qv": case.domain.species["qv"]}, ref)
+
+
+@pytest.mark.parametrize("fct", [True, False])
+def test_fast_division_outside_its_range_matches_oracle(oracle32, fct):
+    """The corrective kernel's three-operation quotient is taken only where
+    numerator and divisor are in range; a wave with an operand outside it
+    computes that phase again with the IEEE division.  Species spanning
+    1e-30..1e25 (numerators below 2^-78 and above 2^50), and, without FCT,
+    species of both signs (divisors r + l + 1e-10 near zero or negative),
+    must still give the oracle's values bit for bit."""
+    case = random_domain(131, 37, 9, 77, 3, fct, True, nspec=4)
+    d = case.domain
+    r = np.random.default_rng(78)
+    names = list(d.species)
+    d.species[names[0]][...] *= np.float32(1e-30)
+    d.species[names[1]][...] *= np.float32(1e25)
+    d.species[names[2]][...] = (10.0 ** r.uniform(-38, 20, d.shape)).astype(np.float32) * \
+        (r.random(d.shape) > 0.3)
+    if not fct:
+        d.species[names[3]][...] -= np.float32(0.5)
+    ref = oracle32.run(d, case.options, case.dt, case.steps, 0, 0, fresh_species(case))
+    assert_same(gpu_run(case, variant=1), ref)

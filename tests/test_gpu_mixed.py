@@ -113,6 +113,18 @@ def run_gpu(case, steps, budget=False):
         st.close()
 
 
+_C4_FP64 = {}
+
+
+def c4_fp64_reference(oracle64, steps):
+    """The FP64 oracle's run of the 256x64x20, 12-species domain (computed
+    once per session: two tests compare with it)."""
+    if steps not in _C4_FP64:
+        base = synthetic.config4(storage=_lib.STORE_BF16, steps=steps)
+        _C4_FP64[steps] = oracle_run_threaded(oracle64, base, steps, fresh_species(base))
+    return _C4_FP64[steps]
+
+
 def oracle_run_threaded(oracle, case, steps, species):
     """The oracle on each species in its own thread (species are independent
     inside advect; the C oracle keeps no state and ctypes releases the GIL)."""
@@ -151,7 +163,7 @@ def test_mixed_c4_error_against_fp64(oracle64, capsys):
     steps = 200
     base = synthetic.config4(storage=_lib.STORE_BF16, steps=steps)
     d = base.domain
-    ref = oracle_run_threaded(oracle64, base, steps, fresh_species(base))
+    ref = c4_fp64_reference(oracle64, steps)
     f64 = lambda a: np.asarray(a, np.float64)
     ref_mass = {n: oracle64.mass(r, f64(d.jacobian), f64(d.density), f64(d.advection_dz))
                 for n, r in ref.items()}
@@ -247,3 +259,110 @@ def test_mixed_closed_box_mass(storage, capsys):
             assert abs(m1[s] - m0[s]) <= 6 * sigma, (n, rel, sigma / abs(m0[s]))
     with capsys.disabled():
         print("\n" + "\n".join(lines))
+
+
+@pytest.mark.parametrize("workload", ["c2", "c5"])
+def test_mixed_crops_error_against_fp64(oracle64, workload, capsys):
+    """Per-field error against the FP64 oracle on crops of the two large
+    workloads, 20 steps, with every species in FP32 and with the
+    hydrometeors stored in 16 bits as the benchmark stores them:
+      c2  130 x 70 x 30 of the 10-species domain, order 3 + FCT; BF16 with
+          stochastic rounding for all but qv and theta;
+      c5  192 x 192 x 50 of the 4096 x 4096 x 50 domain (dx 250 m, jet and
+          ridge), 12 species, order 2 + FCT; ten in BF16.
+    FP32 fields meet 1e-4 relative to max |q|, except theta on c2: at
+    order 3 with FCT its top levels diverge from FP64 in the reference's own
+    FP32 arithmetic (test_precision_spread.py), so there the bound holds
+    below the top 10 levels.  A 16-bit field stays within 4*sqrt(steps)*2^-9.
+    The lines printed are the report."""
+    steps = 20
+    if workload == "c2":
+        def make():
+            c = synthetic.config2(nx=130, ny=70, nz=30, steps=steps)
+            c.options.kernel_variant = 1
+            return c
+    else:
+        make = lambda: synthetic.config5(nx=192, ny=192, nz=50, steps=steps)
+    base = make()
+    d = base.domain
+    base.domain.storage.clear()
+    ref = oracle_run_threaded(oracle64, base, steps, fresh_species(base))
+    f64 = lambda a: np.asarray(a, np.float64)
+    ref_mass = {n: oracle64.mass(r, f64(d.jacobian), f64(d.density), f64(d.advection_dz))
+                for n, r in ref.items()}
+    report = []
+    for mixed in (False, True):
+        case = make()
+        case.domain.storage.clear()
+        if mixed:
+            kind = _lib.STORE_BF16_SR if workload == "c2" else _lib.STORE_BF16
+            for n, a in case.domain.species.items():
+                if n not in ("qv", "theta"):
+                    case.domain.storage[n] = (kind, synthetic.pow2_scale(a, kind))
+        got, mass = run_gpu(case, steps, budget=True)
+        nz = case.domain.nz
+        for n, r in ref.items():
+            g = got[n].astype(np.float64)
+            emax = np.abs(g - r).max() / np.abs(r).max()
+            emean = np.abs(g - r).mean() / np.abs(r).mean()
+            dm = (mass[n] - ref_mass[n]) / abs(ref_mass[n])
+            st = case.domain.storage.get(n, (_lib.STORE_F32, 0))[0]
+            report.append((mixed, n, st, emax, emean, dm))
+            if st == _lib.STORE_F32:
+                if workload == "c2" and n == "theta":
+                    below = np.abs(g[:, :nz - 10] - r[:, :nz - 10]).max() / np.abs(r).max()
+                    assert below <= 1e-4, (mixed, n, below)
+                else:
+                    assert emax <= 1e-4, (mixed, n, emax)
+            else:
+                assert emax <= 4.0 * np.sqrt(steps) * 2.0 ** -9, (mixed, n, emax)
+    with capsys.disabled():
+        for mixed, n, st, emax, emean, dm in report:
+            print(f"\n[precision {workload}] {'mixed' if mixed else 'fp32 '} {n:6s} "
+                  f"{_lib.STORE_NAMES[st]:7s} max {emax:.2e} mean {emean:.2e} budget {dm:+.2e}",
+                  end="")
+        print()
+
+
+def test_compensated_divergence_against_fp64(oracle64, capsys):
+    """The opt-in compensated flux divergence (icar_adv_set_compensation)
+    against the FP64 oracle on the 256x64x20, 12-species domain, 200 steps,
+    with every species in FP32 and with the hydrometeors in BF16 (stochastic
+    rounding), beside the same runs in the reference's operation order.
+    The lines printed are the measurement: per field, the max error
+    relative to max |q| with and without compensation.  Asserted: the
+    compensated runs stay within the same bounds as the plain ones (FP32
+    fields 1e-4, 16-bit fields 4*sqrt(steps)*2^-9), and the FP32 compensated
+    run really differs from the plain one (it is not the reference's
+    rounding)."""
+    steps = 200
+    ref = c4_fp64_reference(oracle64, steps)
+    rows = {}
+    runs = {}
+    for kind in ("f32", "bf16-sr"):
+        for comp in (False, True):
+            case = synthetic.config4(storage=_lib.STORE_BF16_SR, steps=steps)
+            if kind == "f32":
+                case.domain.storage.clear()
+            st = MpdataAdvection(case.domain, case.options)
+            try:
+                st.set_compensation(comp)
+                st.run(case.dt, steps, case.bc_x, case.bc_y)
+                st.download()
+            finally:
+                st.close()
+            got = {n: case.domain.species[n].copy() for n in case.domain.species}
+            runs[(kind, comp)] = got
+            for n, r in ref.items():
+                e = np.abs(got[n].astype(np.float64) - r).max() / np.abs(r).max()
+                rows.setdefault((kind, n), {})[comp] = e
+                st16 = case.domain.storage.get(n, (_lib.STORE_F32, 0))[0]
+                bound = 1e-4 if st16 == _lib.STORE_F32 else 4.0 * np.sqrt(steps) * 2.0 ** -9
+                assert e <= bound, (kind, comp, n, e)
+    assert any(not np.array_equal(runs[("f32", True)][n], runs[("f32", False)][n])
+               for n in ref)
+    with capsys.disabled():
+        for (kind, n), e in rows.items():
+            print(f"\n[compensation] {kind:7s} {n:6s} max err plain {e[False]:.3e} "
+                  f"compensated {e[True]:.3e}", end="")
+        print()
