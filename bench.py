@@ -11,8 +11,8 @@ value = cell-species updates per second = nx*ny*nz*species*steps / t.
 
 Variants on that domain:
   batched_mixed  fused species-batched kernels, qv and theta FP32 and the
-                 ten others stored in BF16 with power-of-two scaling (the
-                 value reported);
+                 ten others stored in BF16 with power-of-two scaling and
+                 stochastic rounding (the value reported);
   batched        the same kernels with every species FP32 (bit-identical to
                  the FP32 oracle);
   plain          one kernel per MPDATA pass and species, every species FP32:
@@ -466,8 +466,8 @@ def c5_line(args) -> dict:
         "parity": "batched and plain bit-identical to the FP32 oracle; batched_mixed equal to the "
                   "FP32 oracle stepped on the 16-bit values with one rounding per step",
         "config": {"workload": f"single domain {d.nx}x{d.ny}x{d.nz}, {S} species (qv, theta FP32; "
-                               "ten BF16 with power-of-two scaling), mpdata_order 2 + FCT, "
-                               "advect_density, Courant 0.5",
+                               "ten BF16 with power-of-two scaling and stochastic rounding), "
+                               "mpdata_order 2 + FCT, advect_density, Courant 0.5",
                    "nx": d.nx, "ny": d.ny, "nz": d.nz, "species": S, "mpdata_order": 2,
                    "fct": True, "variant": "batched_mixed", "dt": case.dt,
                    "generation_s": gen_s, "variants_run": todo},

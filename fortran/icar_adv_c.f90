@@ -16,7 +16,7 @@ module icar_adv_c
     public :: icar_adv_create, icar_adv_destroy, icar_adv_bind_field, icar_adv_bind_species
     public :: icar_adv_upload, icar_adv_download, icar_adv_advect, icar_adv_mass_budget
     public :: icar_adv_max_courant, icar_adv_last_error, icar_adv_sync, icar_adv_apply_bc
-    public :: icar_adv_divergence
+    public :: icar_adv_divergence, icar_adv_set_compensation
     public :: icar_adv_run, icar_adv_set_graph, icar_adv_error_message
     public :: icar_adv_balance_w, icar_adv_compute_dt, icar_adv_launch_config
     public :: icar_adv_set_relax_filter, icar_adv_set_forcing, icar_adv_apply_forcing
@@ -131,6 +131,14 @@ module icar_adv_c
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: handle
             real(c_double), intent(out) :: out(3)
+        end function
+
+        ! the fused kernels' flux divergence with compensated sums (on /= 0)
+        integer(c_int) function icar_adv_set_compensation(handle, on) &
+                bind(C, name="icar_adv_set_compensation")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+            integer(c_int), value :: on
         end function
 
         ! test_divergence (adv_mpdata.f90:445-475) of the last step: out(1) cells

@@ -333,7 +333,7 @@ def dz_levels5(nz: int) -> np.ndarray:
     return 50.0 * 1.07 ** np.arange(nz, dtype=np.float64)
 
 
-def config5(nx=4096, ny=4096, nz=50, steps=20, storage: int = _lib.STORE_BF16,
+def config5(nx=4096, ny=4096, nz=50, steps=20, storage: int = _lib.STORE_BF16_SR,
             workers: int = 16, log=None, scheme: str = "mpdata", names=None) -> Case:
     """Large single domain, batched and mixed precision: dx = 250 m, fractal
     terrain 0-4000 m with a 3000 m ridge line (Gaussian, 5 km half-width, at
@@ -343,8 +343,9 @@ def config5(nx=4096, ny=4096, nz=50, steps=20, storage: int = _lib.STORE_BF16,
     wind) with a perturbation profile that returns aloft what it carries
     near the ground, w from continuity, Courant 0.5;
     the 12 species of config 4, hydrometeor masses and numbers and the tracer
-    stored at 16 bits with per-species power-of-two scaling; order 2 with
-    FCT.  At 4096 x 4096 x 50 (839 M cells) the arrays are filled one level
+    stored at 16 bits with per-species power-of-two scaling (BF16 with
+    stochastic rounding by default: unbiased, so the mass stays within its
+    rounding's random walk); order 2 with FCT.  At 4096 x 4096 x 50 (839 M cells) the arrays are filled one level
     at a time, levels in parallel threads, so that no float64 3-D temporary
     exists; the formulas are those of the other configurations, level by
     level (the random streams are keyed by level).  With scheme "std" the

@@ -269,7 +269,8 @@ def test_mixed_crops_error_against_fp64(oracle64, workload, capsys):
       c2  130 x 70 x 30 of the 10-species domain, order 3 + FCT; BF16 with
           stochastic rounding for all but qv and theta;
       c5  192 x 192 x 50 of the 4096 x 4096 x 50 domain (dx 250 m, jet and
-          ridge), 12 species, order 2 + FCT; ten in BF16.
+          ridge), 12 species, order 2 + FCT; ten in BF16, stochastic
+          rounding.
     FP32 fields meet 1e-4 relative to max |q|, except theta on c2: at
     order 3 with FCT its top levels diverge from FP64 in the reference's own
     FP32 arithmetic (test_precision_spread.py), so there the bound holds
@@ -295,7 +296,7 @@ def test_mixed_crops_error_against_fp64(oracle64, workload, capsys):
         case = make()
         case.domain.storage.clear()
         if mixed:
-            kind = _lib.STORE_BF16_SR if workload == "c2" else _lib.STORE_BF16
+            kind = _lib.STORE_BF16_SR
             for n, a in case.domain.species.items():
                 if n not in ("qv", "theta"):
                     case.domain.storage[n] = (kind, synthetic.pow2_scale(a, kind))

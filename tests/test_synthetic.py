@@ -51,7 +51,7 @@ def test_c5_species_storage_and_courant(c5):
             assert n not in d.storage
         else:
             kind, e = d.storage[n]
-            assert kind == _lib.STORE_BF16 and 0.5 <= np.abs(a).max() / 2.0 ** e <= 1.0, n
+            assert kind == _lib.STORE_BF16_SR and 0.5 <= np.abs(a).max() / 2.0 ** e <= 1.0, n
     dz = d.advection_dz.astype(np.float64)
     c = c5.dt * max(np.abs(d.u).max() / d.dx, np.abs(d.v).max() / d.dx, (np.abs(d.w) / dz).max())
     assert abs(c - 0.5) < 1e-5
