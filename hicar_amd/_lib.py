@@ -45,7 +45,7 @@ EXPORTS = (
     "icar_adv_apply_forcing", "icar_adv_halo_bytes", "icar_adv_halo_pack",
     "icar_adv_halo_unpack", "icar_adv_halo_exchange", "icar_adv_enforce_limits",
     "icar_adv_snapshot", "icar_adv_snapshot_wait", "icar_adv_divergence",
-    "icar_adv_verify_division", "icar_adv_set_compensation",
+    "icar_adv_mass_budget_noalloc", "icar_adv_verify_division", "icar_adv_set_compensation",
 )
 
 
@@ -95,6 +95,8 @@ def load() -> ctypes.CDLL:
         "icar_adv_max_courant": ([vp, dp], ip),
         "icar_adv_divergence": ([vp, dp], ip),
         "icar_adv_set_compensation": ([vp, ip], ip),
+        "icar_adv_mass_budget_noalloc": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp,
+                                          vp, vp, vp], ip),
         "icar_adv_verify_division": ([ctypes.c_uint, ctypes.c_uint,
                                       ctypes.POINTER(ctypes.c_ulonglong),
                                       ctypes.POINTER(ctypes.c_ulonglong)], ip),

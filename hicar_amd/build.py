@@ -34,6 +34,7 @@ UNITS = [
     ("kernels_dyn.hip", ["-ffp-contract=off"]),
     ("kernels_std.hip", ["-ffp-contract=off"]),
     ("api.hip", ["-ffp-contract=off"]),
+    ("noalloc.hip", ["-ffp-contract=off"]),
     ("divcheck.hip", ["-ffp-contract=off"]),
 ]
 

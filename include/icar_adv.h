@@ -194,6 +194,29 @@ int icar_adv_max_courant(void *handle, double out[3]);
  * Synchronises. */
 int icar_adv_divergence(void *handle, double out[2]);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): icar_adv_mass_budget's FP64 sum(q * jacobian * density *
+ * advection_dz) over the interior cells (ims+1:ime-1, all k, jms+1:jme-1) of
+ * one FP32 species, on the caller's device arrays (mass grid, (i, k, j), i
+ * fastest) with the result in *out (device memory, one double), equal bit
+ * for bit to icar_adv_mass_budget's value for the same arrays.  Its per-block
+ * partial sums go to a fixed array of the library on the current device
+ * (ICAR_ADV_NOALLOC_PARTIALS doubles, 8 KiB, held until the process ends)
+ * instead of an allocation.  It enqueues its two kernels on `stream` (a
+ * hipStream_t; NULL is the null stream) and returns: no allocation, memset
+ * or synchronisation, so it can be captured in a graph.  Calls on one GPU
+ * share the array, so calls that overlap (on different streams) race.  The
+ * first call on a device looks up the array's device address
+ * (hipGetSymbolAddress): make it before capturing one.
+ * Errors (icar_adv_last_error(NULL) holds the message): ICAR_ADV_EINVAL for
+ * nx, ny or nz below 3 or a NULL array, ICAR_ADV_ERANGE if the reduction
+ * would need more partial sums than the array holds. */
+#define ICAR_ADV_NOALLOC_PARTIALS 1024
+int icar_adv_mass_budget_noalloc(int nx, int ny, int nz, const float *q, const float *jacobian,
+                                 const float *density, const float *advection_dz, double *out,
+                                 void *stream);
+
 /* = balance_uvw(domain, options) (src/physics/wind.f90:88-164, update
  * =.false.), which the time step runs before advect when advect_density is
  * set (src/main/time_step.f90:324): recomputes the device copy of w from
