@@ -30,8 +30,8 @@ __device__ __forceinline__ void report(unsigned long long *bad, unsigned long lo
 
 // thread t: divisor significand d_first + t / XSPLIT, numerators
 // (t % XSPLIT) + XSPLIT * k; both in [1, 2).  Also, for its divisor, the
-// reciprocal at every exponent e in [-40, 40] against the one at e = 0
-// scaled by 2^-e, and the quotient of -x (once per thread).
+// reciprocal at every exponent e in [FDIV_E_LO, FDIV_E_HI] against the one
+// at e = 0 scaled by 2^-e, and the quotient of -x (once per thread).
 __global__ void __launch_bounds__(CHECK_THREADS)
     k_divcheck(unsigned d_first, unsigned d_count, unsigned long long *bad,
                unsigned long long *first)
@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(CHECK_THREADS)
         }
     }
     if (part == 0) {
-        for (int e = -40; e <= 40; e++) {
+        for (int e = FDIV_E_LO; e <= FDIV_E_HI; e++) {
             const float ds = ldexpf(d, e);
             if (__float_as_uint(recip_nr(ds)) != __float_as_uint(ldexpf(y, -e))) {
                 nbad++;

@@ -41,14 +41,17 @@ __device__ __forceinline__ float div_y(float x, float d, float y)
     return __builtin_fmaf(-rn, y, q0);
 }
 
-// The range the fast quotient is taken in: d in [2^-40, 2^40] (positive), x
-// zero or |x| in [2^-78, 2^50].  Then 1/d, x*y and the quotient are normal,
-// and the remainder is zero or at least 2^-125 (x is a multiple of
-// 2^(ex-23), q0*d of 2^(ex-48)), so it is exact and normal: every step
-// equals, scaled by a power of two, the same step on the operands'
-// significands.
-constexpr float FDIV_D_LO = 0x1p-40f, FDIV_D_HI = 0x1p40f;
-constexpr float FDIV_X_LO = 0x1p-78f, FDIV_X_HI = 0x1p50f;
+// The range the fast quotient is taken in: d in [2^-60, 2^40] (positive,
+// down to the 1e-15 the FCT limiter adds to a zero inflow), x zero or |x|
+// in [2^-78, 2^64].  Then 1/d, x*y and the quotient are normal (|x/d| in
+// [2^-118, 2^124]), and the remainder is zero or at least 2^-125 (x is a
+// multiple of 2^(ex-23), q0*d of 2^(ex-48)), so it is exact and normal:
+// every step equals, scaled by a power of two, the same step on the
+// operands' significands.  The device check covers the reciprocal's scaling
+// over d's exponents FDIV_E_LO..FDIV_E_HI.
+constexpr int FDIV_E_LO = -60, FDIV_E_HI = 40;
+constexpr float FDIV_D_LO = 0x1p-60f, FDIV_D_HI = 0x1p40f;
+constexpr float FDIV_X_LO = 0x1p-78f, FDIV_X_HI = 0x1p64f;
 
 // Guard accumulators of a group of fast quotients, checked once for the
 // group: the smallest and largest divisor, the largest |numerator| and the

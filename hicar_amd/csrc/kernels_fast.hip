@@ -86,9 +86,8 @@ __device__ __forceinline__ uint32_t ld_raw(const void *p, size_t i, int kind)
     return ((gcu32 *)p)[kind == ICAR_ADV_STORE_F32 ? i : (i >> 1)];
 }
 
-// the value of element i from the word ld_raw returned for it
-// the same with a branch on the kind: fewer live registers where the kind is
-// the same for the whole block
+// the value of element i from the word ld_raw returned for it, with a branch
+// on the kind: fewer live registers where the kind is the same for the block
 __device__ __forceinline__ float cvt_q_uniform(uint32_t w, size_t i, int kind, float scale)
 {
     if (kind == ICAR_ADV_STORE_F32) return __uint_as_float(w);
@@ -98,7 +97,8 @@ __device__ __forceinline__ float cvt_q_uniform(uint32_t w, size_t i, int kind, f
     return v * scale;
 }
 
-// (branch-free: both 16-bit forms are decoded and the kind selects)
+// the same without a branch: both 16-bit forms are decoded and the kind
+// selects, so that no conversion waits on a branch while loads are in flight
 __device__ __forceinline__ float cvt_q(uint32_t w, size_t i, int kind, float scale)
 {
     const uint32_t raw = (i & 1) ? (w >> 16) : (w & 0xFFFFu);
@@ -379,7 +379,11 @@ __device__ __forceinline__ float flux1(float l, float r, float U)
     return ((U + fabsf(U)) * l + (U - fabsf(U)) * r) / 2.0f;
 }
 
-template <bool Q16>
+// FD: the two divisions of each species by G and Gz (the same for every
+// species of the cell: one reciprocal each) as the three-operation quotient
+// of fdiv.hpp where the operands are in range, else, for the whole wave, the
+// IEEE division
+template <bool Q16, bool FD>
 __global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ DD, int nd)
 {
     const int dm = find_dom_cb(DD, nd, blockIdx.x);
@@ -407,6 +411,7 @@ __global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ 
         gz = Gz[c];
     }
     const unsigned tstep = *D.step;
+    const float yg = (FD && !ring) ? recip_nr(g) : 1.0f, ygz = (FD && !ring) ? recip_nr(gz) : 1.0f;
     // Species in groups of FSB: every load of a group is issued before the
     // first store, so a thread keeps several species' reads in flight (the
     // compiler cannot move loads above the stores of other species itself:
@@ -447,6 +452,42 @@ __global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ 
             qu[b] = cvt_q(ru[b], c + sk, kind, sc);
             qd[b] = cvt_q(rd[b], c - sk, kind, sc);
         }
+        float qo[FSB];
+        auto update = [&](auto fd_c) -> bool {
+            constexpr bool F = decltype(fd_c)::value;
+            DivGuard gd;
+            if (F) {
+                guard_den(gd, g);
+                guard_den(gd, gz);
+            }
+#pragma unroll
+            for (int b = 0; b < FSB; b++) {
+                qo[b] = qc[b];
+                if (s0 + b >= nspec || ring) continue;  // the ring is copied (:61-63)
+                const float fr = flux1(qc[b], qe[b], Ur), fl = flux1(qw[b], qc[b], Ul);
+                const float fn = flux1(qc[b], qn[b], Vn), fs = flux1(qs[b], qc[b], Vs);
+                const float xh = (fr - fl) + (fn - fs);  // :86-87
+                float xv;
+                if (k == 0)
+                    xv = flux1(qc[b], qu[b], Wt);  // :93-94
+                else if (k == d.nz - 1)
+                    xv = qc[b] * Wt - flux1(qd[b], qc[b], Wb);  // top flux not upwinded (:96-97)
+                else
+                    xv = flux1(qc[b], qu[b], Wt) - flux1(qd[b], qc[b], Wb);  // :90-91
+                if constexpr (F) {
+                    guard_num(gd, xh);
+                    guard_num(gd, xv);
+                    const float q = qc[b] - div_y(xh, g, yg);
+                    qo[b] = q - div_y(xv, gz, ygz);
+                } else {
+                    const float q = qc[b] - xh / g;
+                    qo[b] = q - xv / gz;
+                }
+            }
+            return guard_ok(gd);
+        };
+        if (!FD || __any(!update(std::integral_constant<bool, true>{})))
+            update(std::integral_constant<bool, false>{});
 #pragma unroll
         for (int b = 0; b < FSB; b++) {
             const int s = s0 + b;
@@ -454,20 +495,7 @@ __global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ 
             SpecPass p = sp[s];
             if (!Q16) p.kindo = ICAR_ADV_STORE_F32;
             const uint32_t rnd = is_sr(p.kindo) ? sr_bits((uint32_t)c, s, tstep) : 0u;
-            if (ring) {  // :61-63
-                st_q(p.out, c, p.kindo, p.inv_scaleo, qc[b], rnd);
-                continue;
-            }
-            const float fr = flux1(qc[b], qe[b], Ur), fl = flux1(qw[b], qc[b], Ul);
-            const float fn = flux1(qc[b], qn[b], Vn), fs = flux1(qs[b], qc[b], Vs);
-            float q = qc[b] - ((fr - fl) + (fn - fs)) / g;  // :86-87
-            if (k == 0)
-                q = q - flux1(qc[b], qu[b], Wt) / gz;  // :93-94
-            else if (k == d.nz - 1)
-                q = q - (qc[b] * Wt - flux1(qd[b], qc[b], Wb)) / gz;  // top flux not upwinded (:96-97)
-            else
-                q = q - (flux1(qc[b], qu[b], Wt) - flux1(qd[b], qc[b], Wb)) / gz;  // :90-91
-            st_q(p.out, c, p.kindo, p.inv_scaleo, q, rnd);
+            st_q(p.out, c, p.kindo, p.inv_scaleo, qo[b], rnd);
         }
     }
 }
@@ -476,14 +504,22 @@ __global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ 
 // fused corrective pass
 // ---------------------------------------------------------------------------
 constexpr int CX = 64;            // staged tile width (one wavefront per row)
-constexpr int CY = 20;            // staged tile rows
 constexpr int CH = 2;             // halo
 constexpr int CTI = CX - 2 * CH;  // 60 output columns
-constexpr int CTJ = CY - 2 * CH;  // 16 output rows
-constexpr int NW = 8;             // wavefronts; wave ly stages rows ly, ly+8, ly+16
-constexpr int CNT = 64 * NW;
-constexpr int SR = 3;             // staged-row slots per wave (the third only for ly < 4)
-constexpr int OWN = 2;            // output rows per wave: ly + 8*(rb + t), t = 0, 1
+constexpr int OWN = 2;            // output rows per wave: ly + NW*(rb + t), t = 0, 1
+
+// A corrective tile of TJ output rows (16 or 32): CY = TJ + 4 staged rows,
+// NW = TJ / 2 wavefronts (wave ly stages rows ly, ly + NW, ... and owns two
+// output rows), SR staged-row slots per wave.  Taller tiles recompute fewer
+// halo rows (y faces on CY - 1 rows, y betas on CY - 2) per output row but
+// take twice the LDS (one block of 16 waves per CU instead of two of 8).
+template <int TJ>
+struct Tile {
+    static constexpr int CY = TJ + 2 * CH;
+    static constexpr int NW = TJ / 2;
+    static constexpr int CNT = 64 * NW;
+    static constexpr int SR = (CY + NW - 1) / NW;
+};
 
 
 // s = RN(a + b) and its rounding error e, a + b = s + e exactly (Knuth's
@@ -555,10 +591,11 @@ __device__ __forceinline__ float recip_of(float d, DivGuard &g)
 // per CU need 4); PF: load this iteration's coefficients before the first
 // barrier (latency hidden by the barrier) or at their use; FD: the
 // three-operation quotient where its operands are in range (fdiv.hpp).
-template <bool FCT, int WPE, bool PF, bool Q16, bool FD>
-__global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
+template <bool FCT, int WPE, bool PF, bool Q16, bool FD, int TJ>
+__global__ void __launch_bounds__(Tile<TJ>::CNT) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
     k_corr(const DomDesc *__restrict__ DD, int nd, int pass, unsigned nwork)
 {
+    constexpr int CY = Tile<TJ>::CY, NW = Tile<TJ>::NW, SR = Tile<TJ>::SR;
     // XCD-aware mapping: consecutive work items (the species of one tile)
     // land on the same XCD (blocks are dealt round-robin over 8 XCDs) and
     // share the tile's coefficients in its L2.
@@ -587,7 +624,7 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
     const size_t N = (size_t)nx * ny * nz;
     const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
     const int gi = tx * CTI - CH + lx;
-    const int j0 = ty * CTJ - CH;  // global row of staged row 0
+    const int j0 = ty * TJ - CH;  // global row of staged row 0
     SpecPass p = D.pass[pass - 1][s];
     if (!Q16) p.kind0 = p.kindo = ICAR_ADV_STORE_F32;  // an FP32-only batch: no 16-bit code
     const bool same = p.q1 == nullptr;
@@ -610,13 +647,13 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
     const bool lane_in = gi >= 0 && gi < nx;
     const bool lane_out = lx >= CH && lx < CX - CH && lane_in;
     const int rb = (ly < 2) ? 1 : 0;
-    const int nsr = (ly + 8 * (SR - 1) < CY) ? SR : SR - 1;  // staged rows of this wave
+    const int nsr = (ly + NW * (SR - 1) < CY) ? SR : SR - 1;  // staged rows of this wave
 
     bool row_in[SR];
     unsigned rowbase[SR];  // gi + sj*gj (the level offset is added per use)
 #pragma unroll
     for (int r = 0; r < SR; r++) {
-        const int gj = j0 + ly + 8 * r;
+        const int gj = j0 + ly + NW * r;
         row_in[r] = r < nsr && lane_in && gj >= 0 && gj < ny;
         rowbase[r] = row_in[r] ? (unsigned)gi + sj * (unsigned)gj : 0u;
     }
@@ -665,7 +702,7 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
         float n0[SR];
 #pragma unroll
         for (int r = 0; r < SR; r++) {
-            const int row = ly + 8 * r;
+            const int row = ly + NW * r;
             n0[r] = cvt_q_uniform(n0r[r], rowbase[r] + sk * (unsigned)m, p.kind0, p.scale0);
             if (m < nz && r < nsr) {
                 sq0[m % 3][row][lx] = n0[r];
@@ -760,7 +797,7 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                 const bool kz_ok = L1 <= nz - 2;
 #pragma unroll
                 for (int t = 0; t < OWN; t++) {
-                    const int row = ly + 8 * (rb + t);
+                    const int row = ly + NW * (rb + t);
                     const int gj = j0 + row;
                     float wv = 0.0f, fv = 0.0f;
                     if (kz_ok && lane_out && own_in[t]) {
@@ -795,7 +832,7 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                 // x faces (left of column gi) on the owned rows, lanes 1..63 (:131-165, :378)
 #pragma unroll
                 for (int t = 0; t < OWN; t++) {
-                    const int row = ly + 8 * (rb + t);
+                    const int row = ly + NW * (rb + t);
                     const int gj = j0 + row;
                     float uv = 0.0f, fv = 0.0f;
                     if (lx >= 1 && gi >= 1 && own_in[t]) {
@@ -822,7 +859,7 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                 // y faces (south of row gj) on the staged rows 1..19, lanes 2..61 (:169-205, :379)
 #pragma unroll
                 for (int r4 = 0; r4 < SR; r4++) {
-                    const int row = ly + 8 * r4;
+                    const int row = ly + NW * r4;
                     if (r4 >= nsr || row < 1) continue;
                     const int gj = j0 + row;
                     float vv = 0.0f, fv = 0.0f;
@@ -878,7 +915,7 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                     // x cells of the owned rows, lanes 1..62
 #pragma unroll
                     for (int t = 0; t < OWN; t++) {
-                        const int row = ly + 8 * (rb + t);
+                        const int row = ly + NW * (rb + t);
                         if (lx >= 1 && lx <= CX - 2 && lane_in) {
                             float qmax, qmin, fin = 0.0f, fout = 0.0f;
                             const float qc1 = q1m[row][lx];
@@ -904,7 +941,7 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                     // y cells on the staged rows 1..18, lanes 2..61
 #pragma unroll
                     for (int r4 = 0; r4 < SR; r4++) {
-                        const int row = ly + 8 * r4;
+                        const int row = ly + NW * r4;
                         if (r4 >= nsr || row < 1 || row > CY - 2) continue;
                         const int gj = j0 + row;
                         if (lane_out && gj >= 0 && gj < ny) {
@@ -935,7 +972,7 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                 if (L1 >= 0 && L1 <= nz - 1) {
 #pragma unroll
                     for (int t = 0; t < OWN; t++) {
-                        const int row = ly + 8 * (rb + t);
+                        const int row = ly + NW * (rb + t);
                         const float qc1 = sq1[L1 & 3][row][lx];
                         float qmax, qmin, fin, fout;
                         if (L1 == 0) {
@@ -1001,7 +1038,7 @@ __global__ void __launch_bounds__(CNT) __attribute__((amdgpu_waves_per_eu(WPE, W
                 auto dv = [&](float x, float d) { return fdiv<F>(x, d, g); };
 #pragma unroll
                 for (int t = 0; t < OWN; t++) {
-                    const int row = ly + 8 * (rb + t);
+                    const int row = ly + NW * (rb + t);
                     const int gj = j0 + row;
                     qn[t] = 0.0f;
                     if (lane_out && own_in[t] && L2 >= k0) {
@@ -1104,7 +1141,9 @@ struct FastBatch {
     std::vector<float> dts;              // dt values in the device tables
     unsigned total_cb = 0, total_work = 0;
     double cells = 0, cell_species = 0;  // for the algorithmic byte counts
-    int corr_cfg = 1;    // corrective kernel: bit 0 -> 4 waves/SIMD, bit 1 -> no prefetch
+    int corr_cfg = 1;    // corrective kernel: bit 0 -> 4 waves/SIMD, bit 1 -> no prefetch,
+                         // bit 2 -> 60x32 tiles
+    int tj = 16;         // output rows per corrective tile (tile_rows(corr_cfg))
     int nks = 1;         // requested column chunks along k
     bool tuned = false;  // launch configuration chosen (autotuned or fixed by the environment)
     bool dirty = true;   // device tables out of date
@@ -1295,7 +1334,7 @@ void set_ksplit(FastBatch *b, int nks)
         if (kch < MIN_CHUNK) kch = MIN_CHUNK;
         if (kch > d.nz) kch = d.nz;
         const int chunks = (d.nz + kch - 1) / kch;
-        const int tiles = ((d.nx + CTI - 1) / CTI) * ((d.ny + CTJ - 1) / CTJ);
+        const int tiles = ((d.nx + CTI - 1) / CTI) * ((d.ny + b->tj - 1) / b->tj);
         for (int par = 0; par < 2; par++) {
             DomDesc &E = b->h_desc[par][k];
             E.w0 = w;
@@ -1321,25 +1360,34 @@ int upload_tables(FastBatch *b, std::string *err)
     return 0;
 }
 
+// corr_cfg: bit 0 -> 4 waves per SIMD (else 2), bit 1 -> no prefetch, bit 2 ->
+// 60x32 tiles of 16 waves (which need 4 waves per SIMD: bit 0 is implied)
 template <bool FCT, bool Q16, bool FD>
 void launch_corr_t(unsigned nb, hipStream_t s, const DomDesc *D, int n, int pass, unsigned nw,
                    int cfg)
 {
-    switch (cfg) {
-    case 0: hipLaunchKernelGGL((k_corr<FCT, 2, true, Q16, FD>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-    case 1: hipLaunchKernelGGL((k_corr<FCT, 4, true, Q16, FD>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-    case 2: hipLaunchKernelGGL((k_corr<FCT, 2, false, Q16, FD>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
-    default: hipLaunchKernelGGL((k_corr<FCT, 4, false, Q16, FD>), dim3(nb), dim3(CNT), 0, s, D, n, pass, nw); break;
+    constexpr unsigned T16 = Tile<16>::CNT, T32 = Tile<32>::CNT;
+    switch (cfg & 7) {
+    case 0: hipLaunchKernelGGL((k_corr<FCT, 2, true, Q16, FD, 16>), dim3(nb), dim3(T16), 0, s, D, n, pass, nw); break;
+    case 1: hipLaunchKernelGGL((k_corr<FCT, 4, true, Q16, FD, 16>), dim3(nb), dim3(T16), 0, s, D, n, pass, nw); break;
+    case 2: hipLaunchKernelGGL((k_corr<FCT, 2, false, Q16, FD, 16>), dim3(nb), dim3(T16), 0, s, D, n, pass, nw); break;
+    case 3: hipLaunchKernelGGL((k_corr<FCT, 4, false, Q16, FD, 16>), dim3(nb), dim3(T16), 0, s, D, n, pass, nw); break;
+    case 4:
+    case 5: hipLaunchKernelGGL((k_corr<FCT, 4, true, Q16, FD, 32>), dim3(nb), dim3(T32), 0, s, D, n, pass, nw); break;
+    default: hipLaunchKernelGGL((k_corr<FCT, 4, false, Q16, FD, 32>), dim3(nb), dim3(T32), 0, s, D, n, pass, nw); break;
     }
 }
 
-// ICAR_ADV_IEEE_DIV=1 keeps the IEEE division everywhere in the corrective
-// kernel (the same results; for comparison)
+int tile_rows(int cfg) { return (cfg & 4) ? 32 : 16; }
+
+// ICAR_ADV_FAST_DIV=1 takes the guarded three-operation quotient of fdiv.hpp
+// in the corrective and donor-cell kernels (the same results).  Off by
+// default: its range guard costs about what the cheaper quotient saves.
 bool fast_division()
 {
     static const bool on = [] {
-        const char *e = getenv("ICAR_ADV_IEEE_DIV");
-        return !(e && atoi(e) != 0);
+        const char *e = getenv("ICAR_ADV_FAST_DIV");
+        return e && atoi(e) != 0;
     }();
     return on;
 }
@@ -1385,6 +1433,28 @@ std::string tune_key(const FastBatch *b)
 
 }  // namespace
 
+void launch_courant_b(const FastBatch *b, hipStream_t s, const DomDesc *D, int n)
+{
+    hipLaunchKernelGGL(k_courant_b, dim3(b->total_cb), dim3(64, 4), 0, s, D, n);
+}
+
+void launch_coef(const FastBatch *b, hipStream_t s, const DomDesc *D, int n)
+{
+    hipLaunchKernelGGL(k_coef, dim3(b->total_cb), dim3(64, 4), 0, s, D, n);
+}
+
+void launch_first(const FastBatch *b, hipStream_t s, const DomDesc *D, int n)
+{
+    const bool fd = fast_division();
+    if (b->q16) {
+        if (fd) hipLaunchKernelGGL((k_first<true, true>), dim3(b->total_cb), dim3(FBX, FBY), 0, s, D, n);
+        else hipLaunchKernelGGL((k_first<true, false>), dim3(b->total_cb), dim3(FBX, FBY), 0, s, D, n);
+    } else {
+        if (fd) hipLaunchKernelGGL((k_first<false, true>), dim3(b->total_cb), dim3(FBX, FBY), 0, s, D, n);
+        else hipLaunchKernelGGL((k_first<false, false>), dim3(b->total_cb), dim3(FBX, FBY), 0, s, D, n);
+    }
+}
+
 int fast_batch_tune(FastBatch *b, hipStream_t s, std::string *err)
 {
     if (b->tuned) return 0;
@@ -1400,39 +1470,61 @@ int fast_batch_tune(FastBatch *b, hipStream_t s, std::string *err)
         auto it = g_tuned.find(key);
         if (it != g_tuned.end()) {
             b->corr_cfg = it->second.first;
+            b->tj = tile_rows(b->corr_cfg);
             set_ksplit(b, it->second.second);
             b->tuned = true;
             return 0;
         }
     }
-    // candidates: the launch configurations, and column chunks while one
-    // chunk per column leaves the GPU with few blocks
-    std::vector<int> cfgs = fix_cfg ? std::vector<int>{b->corr_cfg} : std::vector<int>{1, 0, 3, 2};
-    std::vector<int> kss{b->nks};
-    if (!fix_ks) {
-        set_ksplit(b, 1);
-        const unsigned base = b->total_work;
-        kss = {1};
-        if (base < 4096)
-            for (int k : {2, 3, 4, 6, 8})
-                if ((unsigned)k * base <= 16384) kss.push_back(k);
-    }
+    // candidates: the tile heights and launch configurations, and column
+    // chunks while one chunk per column leaves the GPU with few blocks
+    std::vector<int> cfgs = fix_cfg ? std::vector<int>{b->corr_cfg}
+                                    : std::vector<int>{1, 0, 3, 2, 5, 7};
     const int n = (int)b->doms.size();
     const int par = b->doms[0].st->cur;
     // The timed launches are pass 2, which writes only the FP32 pass buffer
     // (order >= 3) or the alternate home buffer (order 2): both are written
-    // again by the step that follows before anything reads them.
+    // again by the step that follows before anything reads them.  Its inputs
+    // are made first, as the step makes them (Courant numbers, coefficients
+    // and the donor-cell pass of the current state): timed on the real data,
+    // the candidates take the path the step will take.
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return 0;
     if (hipEventCreate(&e1) != hipSuccess) {
         (void)hipEventDestroy(e0);
         return 0;
     }
-    float best = 1e30f;
-    int best_cfg = b->corr_cfg, best_ks = kss[0];
     int rc = 0;
-    std::vector<std::vector<int>> seen;  // chunk counts already timed
-    for (int ks : kss) {
+    if (b->dirty && (rc = upload_tables(b, err))) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return rc;
+    }
+    {
+        const DomDesc *D0 = b->d_desc + (size_t)par * n;
+        launch_courant_b(b, s, D0, n);
+        launch_coef(b, s, D0, n);
+        launch_first(b, s, D0, n);
+    }
+    float best = 1e30f;
+    int best_cfg = b->corr_cfg, best_ks = b->nks;
+    for (int tj : {16, 32}) {
+      std::vector<int> tcfgs;
+      for (int cfg : cfgs)
+          if (tile_rows(cfg) == tj) tcfgs.push_back(cfg);
+      if (tcfgs.empty()) continue;
+      b->tj = tj;
+      std::vector<int> kss{b->nks};
+      if (!fix_ks) {
+          set_ksplit(b, 1);
+          const unsigned base = b->total_work;
+          kss = {1};
+          if (base < 4096)
+              for (int k : {2, 3, 4, 6, 8})
+                  if ((unsigned)k * base <= 16384) kss.push_back(k);
+      }
+      std::vector<std::vector<int>> seen;  // chunk counts already timed
+      for (int ks : kss) {
         set_ksplit(b, ks);
         std::vector<int> sig;
         for (int k = 0; k < n; k++) sig.push_back(b->h_desc[0][k].nks);
@@ -1442,7 +1534,7 @@ int fast_batch_tune(FastBatch *b, hipStream_t s, std::string *err)
         seen.push_back(sig);
         if ((rc = upload_tables(b, err))) break;
         const DomDesc *D = b->d_desc + (size_t)par * n;
-        for (int cfg : cfgs) {
+        for (int cfg : tcfgs) {
             launch_corr(b, s, D, n, 2, cfg);  // warm-up
             (void)hipEventRecord(e0, s);
             for (int r = 0; r < 2; r++) launch_corr(b, s, D, n, 2, cfg);
@@ -1461,11 +1553,14 @@ int fast_batch_tune(FastBatch *b, hipStream_t s, std::string *err)
             }
         }
         if (rc) break;
+      }
+      if (rc) break;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     b->corr_cfg = best_cfg;
+    b->tj = tile_rows(best_cfg);
     set_ksplit(b, best_ks);
     b->tuned = true;
     if (cached) {
@@ -1491,12 +1586,13 @@ FastBatch *fast_batch_create(const FastDomain *doms, int n, int order, int fct, 
     b->doms.assign(doms, doms + n);
     b->order = order;
     b->fct = fct;
-    // launch configuration: ICAR_ADV_CORR_CFG (0..3) and ICAR_ADV_KSPLIT fix
+    // launch configuration: ICAR_ADV_CORR_CFG (0..7) and ICAR_ADV_KSPLIT fix
     // it, ICAR_ADV_AUTOTUNE=0 keeps the defaults; otherwise the first step
     // times the candidates on this batch
     const char *ec = getenv("ICAR_ADV_CORR_CFG"), *ek = getenv("ICAR_ADV_KSPLIT");
     const char *ea = getenv("ICAR_ADV_AUTOTUNE");
-    if (ec) b->corr_cfg = atoi(ec) & 3;
+    if (ec) b->corr_cfg = atoi(ec) & 7;
+    b->tj = tile_rows(b->corr_cfg);
     if (ek) b->nks = atoi(ek) < 1 ? 1 : atoi(ek);
     b->tuned = (ec && ek) || (ea && atoi(ea) == 0) || order < 2;
     unsigned cb = 0;
@@ -1607,19 +1703,11 @@ int fast_batch_advect(FastBatch *b, hipStream_t s, const float *dt, const TimedF
     const DomDesc *D = b->d_desc + (size_t)par * n;
     const double cells = b->cells, cs = b->cell_species;
     // Courant numbers: reads u, v, w, density, jacobians, dz; writes 6 fields
-    timed(K_COURANT, (9.0 + 6.0) * 4 * cells,
-          [&] { hipLaunchKernelGGL(k_courant_b, dim3(b->total_cb), dim3(64, 4), 0, s, D, n); });
+    timed(K_COURANT, (9.0 + 6.0) * 4 * cells, [&] { launch_courant_b(b, s, D, n); });
     // species-invariant coefficients: reads U, V, Wd, G; writes NCOEF
-    timed(K_COEF, (4.0 + NCOEF) * 4 * cells,
-          [&] { hipLaunchKernelGGL(k_coef, dim3(b->total_cb), dim3(64, 4), 0, s, D, n); });
+    timed(K_COEF, (4.0 + NCOEF) * 4 * cells, [&] { launch_coef(b, s, D, n); });
     // pass 1: donor cell for every species
-    timed(K_FUSED_FIRST, 5.0 * 4 * cells + 2.0 * 4 * cs,
-          [&] {
-              if (b->q16)
-                  hipLaunchKernelGGL(k_first<true>, dim3(b->total_cb), dim3(FBX, FBY), 0, s, D, n);
-              else
-                  hipLaunchKernelGGL(k_first<false>, dim3(b->total_cb), dim3(FBX, FBY), 0, s, D, n);
-          });
+    timed(K_FUSED_FIRST, 5.0 * 4 * cells + 2.0 * 4 * cs, [&] { launch_first(b, s, D, n); });
     for (int pass = 2; pass <= b->order; pass++) {
         // species read (q0 and q1 at pass 2) and written once; coefficients,
         // G, Gz and dz once

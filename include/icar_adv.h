@@ -335,11 +335,14 @@ int icar_adv_kernel_stats(void *handle, int id, double *total_ms, long long *lau
 /* Launch configuration of the fused corrective kernel chosen for this handle
  * (kernel_variant 1), -1 before the first advect.  The first fused step times
  * the candidates on the handle's own grid and species count and keeps the
- * fastest (cached for the process by shape): corr_cfg 0..3 selects the
- * register budget (2 or 4 waves per SIMD) and whether the next level is
- * prefetched; ksplit is the number of chunks each column march is split into
- * (more blocks for small grids).  The environment variables ICAR_ADV_CORR_CFG
- * and ICAR_ADV_KSPLIT fix them, ICAR_ADV_AUTOTUNE=0 keeps 1 and 1. */
+ * fastest (cached for the process by shape).  The tile is a column of 60 x 16
+ * or 60 x 32 cells with a halo of 2, marched up through the levels; corr_cfg
+ * bit 0 selects the register budget (4 waves per SIMD, else 2), bit 1 no
+ * prefetch of the next level, bit 2 the 60 x 32 tile (blocks of 16 waves,
+ * always 4 per SIMD); ksplit is the number of chunks each column march is
+ * split into, i.e. the tile's extent in k (more blocks for small grids).  The
+ * environment variables ICAR_ADV_CORR_CFG (0..7) and ICAR_ADV_KSPLIT fix
+ * them, ICAR_ADV_AUTOTUNE=0 keeps 1 and 1. */
 int icar_adv_launch_config(void *handle, int *corr_cfg, int *ksplit);
 
 /* Records the nsteps loop of icar_adv_run in a HIP graph on its first call
@@ -362,15 +365,16 @@ int icar_adv_set_graph(void *handle, int enable);
 int icar_adv_set_compensation(void *handle, int on);
 
 /* Checks, on the current device, the three-operation quotient the kernels
- * take in place of an IEEE division where both operands are in range (q0 =
+ * take in place of an IEEE division where both operands are in range, when
+ * the environment sets ICAR_ADV_FAST_DIV=1 (off by default) (q0 =
  * x*y, rn = q0*d - x, q = q0 - rn*y with y the hardware reciprocal of d
  * refined by one Newton step): for every divisor significand in [d_first,
  * d_first + d_count) (d in [1, 2); d_first + d_count <= 2^23) and every
  * numerator significand (x in [1, 2)) it must equal x / d bit for bit, the
- * reciprocal must scale exactly with d's exponent over [-40, 40], and a
+ * reciprocal must scale exactly with d's exponent over [-60, 40], and a
  * negative numerator must give the negated quotient.  *mismatches gets the
  * number of failures, *first the first found ((d bits << 32) | x bits, or
- * all ones).  All 2^23 divisors take about a minute.  Synchronises. */
+ * all ones).  All 2^23 divisors take about 30 s.  Synchronises. */
 int icar_adv_verify_division(unsigned d_first, unsigned d_count, unsigned long long *mismatches,
                              unsigned long long *first);
 

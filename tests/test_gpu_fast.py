@@ -147,8 +147,11 @@ def test_column_chunks_match_oracle(oracle32, launch_env, ksplit, order, fct):
     assert_same({n: case.domain.species[n] for n in ref}, ref)
 
 
-@pytest.mark.parametrize("cfg", [0, 2, 3])
+@pytest.mark.parametrize("cfg", [0, 2, 3, 5, 7])
 def test_launch_configurations_match_oracle(oracle32, launch_env, cfg):
+    """Every corrective-kernel configuration, the 60x32 tiles (cfg 5, 7)
+    included, on a domain whose 37 rows leave partial tiles of either
+    height."""
     launch_env(cfg=cfg, ksplit=1)
     case = random_domain(131, 37, 9, 400 + cfg, 3, True, True, nspec=2)
     ref = oracle32.run(case.domain, case.options, case.dt, case.steps, 0, 0, fresh_species(case))
@@ -173,19 +176,19 @@ def test_autotuned_small_domain_matches_oracle(oracle32, launch_env):
         cfg, ks = st.launch_config()
     finally:
         st.close()
-    assert 0 <= cfg <= 3 and 1 <= ks <= 5
+    assert 0 <= cfg <= 7 and 1 <= ks <= 5
     print(f"\n[autotune] 256x64x20, 1 species: corr_cfg {cfg}, column chunks {ks}")
     assert_same({"qv": case.domain.species["qv"]}, ref)
 
 
 @pytest.mark.parametrize("fct", [True, False])
 def test_fast_division_outside_its_range_matches_oracle(oracle32, fct):
-    """The corrective kernel's three-operation quotient is taken only where
-    numerator and divisor are in range; a wave with an operand outside it
-    computes that phase again with the IEEE division.  Species spanning
-    1e-30..1e25 (numerators below 2^-78 and above 2^50), and, without FCT,
-    species of both signs (divisors r + l + 1e-10 near zero or negative),
-    must still give the oracle's values bit for bit."""
+    """Species spanning 1e-30..1e25 (numerators below 2^-78 and above 2^64),
+    and, without FCT, species of both signs (divisors r + l + 1e-10 near zero
+    or negative) give the oracle's values bit for bit.  With the opt-in
+    three-operation quotient (ICAR_ADV_FAST_DIV=1) such operands are outside
+    its range and the waves holding them take the IEEE division; the same
+    case runs under that setting in tests/fastdiv_check.py."""
     case = random_domain(131, 37, 9, 77, 3, fct, True, nspec=4)
     d = case.domain
     r = np.random.default_rng(78)
