@@ -177,6 +177,73 @@ __device__ __forceinline__ bool is_sr(int kind)
     return kind == ICAR_ADV_STORE_BF16_SR || kind == ICAR_ADV_STORE_F16_SR;
 }
 
+// The storage kinds a kernel is compiled for: FP32 only; FP32 and the BF16
+// kinds; any kind.  A species' kind is the same for a whole block (or, in the
+// donor-cell kernel, for a species of the loop), so the BF16 class branches on
+// it around its loads: one 16-bit load of a BF16 element (zero-extended) or
+// one 32-bit load of an FP32 one.  The branch holds the load only, so that the
+// load stays in flight past it; the conversion (a shift into the high half
+// and the scale) comes where the value is used.  No FP16 path.
+constexpr int QK_F32 = 0, QK_BF16 = 1, QK_ANY = 2;
+
+// the raw word of element i: the float's bits (FP32), the 16 bits (BF16
+// class, a 16-bit kind), or the ld_raw word (any kind)
+template <int QK>
+__device__ __forceinline__ uint32_t ld_w(const void *p, size_t i, int kind)
+{
+    if (QK == QK_F32) return ((gcu32 *)p)[i];
+    if (QK == QK_BF16) {
+        if (kind == ICAR_ADV_STORE_F32) return ((gcu32 *)p)[i];
+        return ((gcu16 *)p)[i];
+    }
+    return ld_raw(p, i, kind);
+}
+
+// the value of element i from its ld_w word
+template <int QK>
+__device__ __forceinline__ float cv_w(uint32_t w, size_t i, int kind, float scale)
+{
+    if (QK == QK_F32) return __uint_as_float(w);
+    if (QK == QK_BF16)
+        return kind == ICAR_ADV_STORE_F32 ? __uint_as_float(w) : __uint_as_float(w << 16) * scale;
+    return cvt_q(w, i, kind, scale);
+}
+
+// the same where the kind is uniform for the block (any kind: a branch, not a
+// select)
+template <int QK>
+__device__ __forceinline__ float cv_w_uniform(uint32_t w, size_t i, int kind, float scale)
+{
+    if (QK == QK_ANY) return cvt_q_uniform(w, i, kind, scale);
+    return cv_w<QK>(w, i, kind, scale);
+}
+
+// BF16 bits of x: stochastic rounding with the 16 random bits rnd (sr), else
+// round to nearest even; infinities and NaN are kept (to16)
+__device__ __forceinline__ uint16_t bf16_of(float x, bool sr, uint32_t rnd)
+{
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += sr ? rnd : 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+template <int QK>
+__device__ __forceinline__ void st_w(void *p, size_t i, int kind, float inv_scale, float v,
+                                     uint32_t rnd)
+{
+    if (QK == QK_F32) {
+        ((gfloat *)p)[i] = v;
+    } else if (QK == QK_BF16) {
+        if (kind == ICAR_ADV_STORE_F32)
+            ((gfloat *)p)[i] = v;
+        else
+            ((gu16 *)p)[i] = bf16_of(v * inv_scale, kind == ICAR_ADV_STORE_BF16_SR, rnd);
+    } else {
+        st_q(p, i, kind, inv_scale, v, rnd);
+    }
+}
+
 __device__ __forceinline__ float max3(float a, float b, float c) { return fmaxf(fmaxf(a, b), c); }
 __device__ __forceinline__ float min3(float a, float b, float c) { return fminf(fminf(a, b), c); }
 
@@ -383,7 +450,7 @@ __device__ __forceinline__ float flux1(float l, float r, float U)
 // species of the cell: one reciprocal each) as the three-operation quotient
 // of fdiv.hpp where the operands are in range, else, for the whole wave, the
 // IEEE division
-template <bool Q16, bool FD>
+template <int QK, bool FD>
 __global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ DD, int nd)
 {
     const int dm = find_dom_cb(DD, nd, blockIdx.x);
@@ -423,15 +490,15 @@ __global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ 
             rc[b] = rw[b] = re[b] = rs[b] = rn[b] = ru[b] = rd[b] = 0u;
             if (s0 + b < nspec) {
                 SpecPass p = sp[s0 + b];
-                if (!Q16) p.kind0 = ICAR_ADV_STORE_F32;  // an FP32-only batch: no 16-bit code
-                rc[b] = ld_raw(p.q0, c, p.kind0);
+                if (QK == QK_F32) p.kind0 = ICAR_ADV_STORE_F32;  // an FP32-only batch: no 16-bit code
+                rc[b] = ld_w<QK>(p.q0, c, p.kind0);
                 if (!ring) {
-                    rw[b] = ld_raw(p.q0, c - 1, p.kind0);
-                    re[b] = ld_raw(p.q0, c + 1, p.kind0);
-                    rs[b] = ld_raw(p.q0, c - sj, p.kind0);
-                    rn[b] = ld_raw(p.q0, c + sj, p.kind0);
-                    if (k < d.nz - 1) ru[b] = ld_raw(p.q0, c + sk, p.kind0);
-                    if (k > 0) rd[b] = ld_raw(p.q0, c - sk, p.kind0);
+                    rw[b] = ld_w<QK>(p.q0, c - 1, p.kind0);
+                    re[b] = ld_w<QK>(p.q0, c + 1, p.kind0);
+                    rs[b] = ld_w<QK>(p.q0, c - sj, p.kind0);
+                    rn[b] = ld_w<QK>(p.q0, c + sj, p.kind0);
+                    if (k < d.nz - 1) ru[b] = ld_w<QK>(p.q0, c + sk, p.kind0);
+                    if (k > 0) rd[b] = ld_w<QK>(p.q0, c - sk, p.kind0);
                 }
             }
         }
@@ -440,17 +507,17 @@ __global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ 
         for (int b = 0; b < FSB; b++) {
             int kind = ICAR_ADV_STORE_F32;
             float sc = 1.0f;
-            if (Q16 && s0 + b < nspec) {
+            if (QK != QK_F32 && s0 + b < nspec) {
                 kind = sp[s0 + b].kind0;
                 sc = sp[s0 + b].scale0;
             }
-            qc[b] = cvt_q(rc[b], c, kind, sc);
-            qw[b] = cvt_q(rw[b], c - 1, kind, sc);
-            qe[b] = cvt_q(re[b], c + 1, kind, sc);
-            qs[b] = cvt_q(rs[b], c - sj, kind, sc);
-            qn[b] = cvt_q(rn[b], c + sj, kind, sc);
-            qu[b] = cvt_q(ru[b], c + sk, kind, sc);
-            qd[b] = cvt_q(rd[b], c - sk, kind, sc);
+            qc[b] = cv_w<QK>(rc[b], c, kind, sc);
+            qw[b] = cv_w<QK>(rw[b], c - 1, kind, sc);
+            qe[b] = cv_w<QK>(re[b], c + 1, kind, sc);
+            qs[b] = cv_w<QK>(rs[b], c - sj, kind, sc);
+            qn[b] = cv_w<QK>(rn[b], c + sj, kind, sc);
+            qu[b] = cv_w<QK>(ru[b], c + sk, kind, sc);
+            qd[b] = cv_w<QK>(rd[b], c - sk, kind, sc);
         }
         float qo[FSB];
         auto update = [&](auto fd_c) -> bool {
@@ -493,9 +560,9 @@ __global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ 
             const int s = s0 + b;
             if (s >= nspec) continue;
             SpecPass p = sp[s];
-            if (!Q16) p.kindo = ICAR_ADV_STORE_F32;
+            if (QK == QK_F32) p.kindo = ICAR_ADV_STORE_F32;
             const uint32_t rnd = is_sr(p.kindo) ? sr_bits((uint32_t)c, s, tstep) : 0u;
-            st_q(p.out, c, p.kindo, p.inv_scaleo, qo[b], rnd);
+            st_w<QK>(p.out, c, p.kindo, p.inv_scaleo, qo[b], rnd);
         }
     }
 }
@@ -591,7 +658,7 @@ __device__ __forceinline__ float recip_of(float d, DivGuard &g)
 // per CU need 4); PF: load this iteration's coefficients before the first
 // barrier (latency hidden by the barrier) or at their use; FD: the
 // three-operation quotient where its operands are in range (fdiv.hpp).
-template <bool FCT, int WPE, bool PF, bool Q16, bool FD, int TJ>
+template <bool FCT, int WPE, bool PF, int QK, bool FD, int TJ>
 __global__ void __launch_bounds__(Tile<TJ>::CNT) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
     k_corr(const DomDesc *__restrict__ DD, int nd, int pass, unsigned nwork)
 {
@@ -626,7 +693,7 @@ __global__ void __launch_bounds__(Tile<TJ>::CNT) __attribute__((amdgpu_waves_per
     const int gi = tx * CTI - CH + lx;
     const int j0 = ty * TJ - CH;  // global row of staged row 0
     SpecPass p = D.pass[pass - 1][s];
-    if (!Q16) p.kind0 = p.kindo = ICAR_ADV_STORE_F32;  // an FP32-only batch: no 16-bit code
+    if (QK == QK_F32) p.kind0 = p.kindo = ICAR_ADV_STORE_F32;  // an FP32-only batch: no 16-bit code
     const bool same = p.q1 == nullptr;
     const unsigned tstep = is_sr(p.kindo) ? *D.step : 0u;
     const float eps = 1e-10f, eps15 = 1e-15f;
@@ -690,7 +757,7 @@ __global__ void __launch_bounds__(Tile<TJ>::CNT) __attribute__((amdgpu_waves_per
         n1[r] = 0.0f;
         if (row_in[r]) {
             const unsigned c = rowbase[r] + sk * (unsigned)m0;
-            n0r[r] = ld_raw(p.q0, c, p.kind0);
+            n0r[r] = ld_w<QK>(p.q0, c, p.kind0);
             if (!same) n1[r] = q1p[c];
         }
     }
@@ -703,7 +770,7 @@ __global__ void __launch_bounds__(Tile<TJ>::CNT) __attribute__((amdgpu_waves_per
 #pragma unroll
         for (int r = 0; r < SR; r++) {
             const int row = ly + NW * r;
-            n0[r] = cvt_q_uniform(n0r[r], rowbase[r] + sk * (unsigned)m, p.kind0, p.scale0);
+            n0[r] = cv_w_uniform<QK>(n0r[r], rowbase[r] + sk * (unsigned)m, p.kind0, p.scale0);
             if (m < nz && r < nsr) {
                 sq0[m % 3][row][lx] = n0[r];
                 sq1[m & 3][row][lx] = same ? n0[r] : n1[r];
@@ -721,7 +788,7 @@ __global__ void __launch_bounds__(Tile<TJ>::CNT) __attribute__((amdgpu_waves_per
             for (int r = 0; r < SR; r++) {
                 if (row_in[r]) {
                     const unsigned c = rowbase[r] + sk * (unsigned)(m + 1);
-                    n0r[r] = ld_raw(p.q0, c, p.kind0);
+                    n0r[r] = ld_w<QK>(p.q0, c, p.kind0);
                     if (!same) n1[r] = q1p[c];
                 }
             }
@@ -1095,8 +1162,8 @@ __global__ void __launch_bounds__(Tile<TJ>::CNT) __attribute__((amdgpu_waves_per
             for (int t = 0; t < OWN; t++) {
                 if (lane_out && own_in[t] && L2 >= k0) {
                     const unsigned c = own_base[t] + sk * (unsigned)L2;
-                    st_q(p.out, c, p.kindo, p.inv_scaleo, qn[t],
-                         is_sr(p.kindo) ? sr_bits(c, s, tstep) : 0u);
+                    st_w<QK>(p.out, c, p.kindo, p.inv_scaleo, qn[t],
+                             is_sr(p.kindo) ? sr_bits(c, s, tstep) : 0u);
                 }
                 Fzp[t] = Fzc[t];
             }
@@ -1149,6 +1216,7 @@ struct FastBatch {
     bool dirty = true;   // device tables out of date
     bool has_sr = false; // a species is stored with stochastic rounding
     bool q16 = false;    // a species is stored in 16 bits
+    bool f16 = false;    // a species is stored in FP16 (kernels for any kind)
 };
 
 namespace {
@@ -1362,19 +1430,36 @@ int upload_tables(FastBatch *b, std::string *err)
 
 // corr_cfg: bit 0 -> 4 waves per SIMD (else 2), bit 1 -> no prefetch, bit 2 ->
 // 60x32 tiles of 16 waves (which need 4 waves per SIMD: bit 0 is implied)
-template <bool FCT, bool Q16, bool FD>
+template <bool FCT, int QK, bool FD>
 void launch_corr_t(unsigned nb, hipStream_t s, const DomDesc *D, int n, int pass, unsigned nw,
                    int cfg)
 {
     constexpr unsigned T16 = Tile<16>::CNT, T32 = Tile<32>::CNT;
     switch (cfg & 7) {
-    case 0: hipLaunchKernelGGL((k_corr<FCT, 2, true, Q16, FD, 16>), dim3(nb), dim3(T16), 0, s, D, n, pass, nw); break;
-    case 1: hipLaunchKernelGGL((k_corr<FCT, 4, true, Q16, FD, 16>), dim3(nb), dim3(T16), 0, s, D, n, pass, nw); break;
-    case 2: hipLaunchKernelGGL((k_corr<FCT, 2, false, Q16, FD, 16>), dim3(nb), dim3(T16), 0, s, D, n, pass, nw); break;
-    case 3: hipLaunchKernelGGL((k_corr<FCT, 4, false, Q16, FD, 16>), dim3(nb), dim3(T16), 0, s, D, n, pass, nw); break;
+    case 0: hipLaunchKernelGGL((k_corr<FCT, 2, true, QK, FD, 16>), dim3(nb), dim3(T16), 0, s, D, n, pass, nw); break;
+    case 1: hipLaunchKernelGGL((k_corr<FCT, 4, true, QK, FD, 16>), dim3(nb), dim3(T16), 0, s, D, n, pass, nw); break;
+    case 2: hipLaunchKernelGGL((k_corr<FCT, 2, false, QK, FD, 16>), dim3(nb), dim3(T16), 0, s, D, n, pass, nw); break;
+    case 3: hipLaunchKernelGGL((k_corr<FCT, 4, false, QK, FD, 16>), dim3(nb), dim3(T16), 0, s, D, n, pass, nw); break;
     case 4:
-    case 5: hipLaunchKernelGGL((k_corr<FCT, 4, true, Q16, FD, 32>), dim3(nb), dim3(T32), 0, s, D, n, pass, nw); break;
-    default: hipLaunchKernelGGL((k_corr<FCT, 4, false, Q16, FD, 32>), dim3(nb), dim3(T32), 0, s, D, n, pass, nw); break;
+    case 5: hipLaunchKernelGGL((k_corr<FCT, 4, true, QK, FD, 32>), dim3(nb), dim3(T32), 0, s, D, n, pass, nw); break;
+    default: hipLaunchKernelGGL((k_corr<FCT, 4, false, QK, FD, 32>), dim3(nb), dim3(T32), 0, s, D, n, pass, nw); break;
+    }
+}
+
+// the kernels' storage class for a batch: FP32 only, FP32 and BF16, or any kind
+int storage_class(const FastBatch *b)
+{
+    return !b->q16 ? QK_F32 : b->f16 ? QK_ANY : QK_BF16;
+}
+
+template <bool FCT, bool FD>
+void launch_corr_q(const FastBatch *b, unsigned nb, hipStream_t s, const DomDesc *D, int n,
+                   int pass, unsigned nw, int cfg)
+{
+    switch (storage_class(b)) {
+    case QK_F32: launch_corr_t<FCT, QK_F32, FD>(nb, s, D, n, pass, nw, cfg); break;
+    case QK_BF16: launch_corr_t<FCT, QK_BF16, FD>(nb, s, D, n, pass, nw, cfg); break;
+    default: launch_corr_t<FCT, QK_ANY, FD>(nb, s, D, n, pass, nw, cfg); break;
     }
 }
 
@@ -1398,21 +1483,11 @@ void launch_corr(const FastBatch *b, hipStream_t s, const DomDesc *D, int n, int
     const unsigned nb = (nw + 7) / 8 * 8;
     const bool fd = fast_division();
     if (b->fct) {
-        if (b->q16) {
-            if (fd) launch_corr_t<true, true, true>(nb, s, D, n, pass, nw, cfg);
-            else launch_corr_t<true, true, false>(nb, s, D, n, pass, nw, cfg);
-        } else {
-            if (fd) launch_corr_t<true, false, true>(nb, s, D, n, pass, nw, cfg);
-            else launch_corr_t<true, false, false>(nb, s, D, n, pass, nw, cfg);
-        }
+        if (fd) launch_corr_q<true, true>(b, nb, s, D, n, pass, nw, cfg);
+        else launch_corr_q<true, false>(b, nb, s, D, n, pass, nw, cfg);
     } else {
-        if (b->q16) {
-            if (fd) launch_corr_t<false, true, true>(nb, s, D, n, pass, nw, cfg);
-            else launch_corr_t<false, true, false>(nb, s, D, n, pass, nw, cfg);
-        } else {
-            if (fd) launch_corr_t<false, false, true>(nb, s, D, n, pass, nw, cfg);
-            else launch_corr_t<false, false, false>(nb, s, D, n, pass, nw, cfg);
-        }
+        if (fd) launch_corr_q<false, true>(b, nb, s, D, n, pass, nw, cfg);
+        else launch_corr_q<false, false>(b, nb, s, D, n, pass, nw, cfg);
     }
 }
 
@@ -1422,9 +1497,9 @@ std::map<std::string, std::pair<int, int>> g_tuned;
 
 std::string tune_key(const FastBatch *b)
 {
-    // the 16-bit and FP32-only batches run different kernel instantiations
+    // batches of different storage classes run different kernel instantiations
     std::string k = std::to_string(b->order) + "/" + std::to_string(b->fct) + "/" +
-                    std::to_string((int)b->q16);
+                    std::to_string(storage_class(b));
     for (const auto &dm : b->doms)
         k += ":" + std::to_string(dm.st->d.nx) + "x" + std::to_string(dm.st->d.ny) + "x" +
              std::to_string(dm.st->d.nz) + "s" + std::to_string(dm.st->sp.size());
@@ -1446,12 +1521,20 @@ void launch_coef(const FastBatch *b, hipStream_t s, const DomDesc *D, int n)
 void launch_first(const FastBatch *b, hipStream_t s, const DomDesc *D, int n)
 {
     const bool fd = fast_division();
-    if (b->q16) {
-        if (fd) hipLaunchKernelGGL((k_first<true, true>), dim3(b->total_cb), dim3(FBX, FBY), 0, s, D, n);
-        else hipLaunchKernelGGL((k_first<true, false>), dim3(b->total_cb), dim3(FBX, FBY), 0, s, D, n);
-    } else {
-        if (fd) hipLaunchKernelGGL((k_first<false, true>), dim3(b->total_cb), dim3(FBX, FBY), 0, s, D, n);
-        else hipLaunchKernelGGL((k_first<false, false>), dim3(b->total_cb), dim3(FBX, FBY), 0, s, D, n);
+    const dim3 g(b->total_cb), t(FBX, FBY);
+    switch (storage_class(b)) {
+    case QK_F32:
+        if (fd) hipLaunchKernelGGL((k_first<QK_F32, true>), g, t, 0, s, D, n);
+        else hipLaunchKernelGGL((k_first<QK_F32, false>), g, t, 0, s, D, n);
+        break;
+    case QK_BF16:
+        if (fd) hipLaunchKernelGGL((k_first<QK_BF16, true>), g, t, 0, s, D, n);
+        else hipLaunchKernelGGL((k_first<QK_BF16, false>), g, t, 0, s, D, n);
+        break;
+    default:
+        if (fd) hipLaunchKernelGGL((k_first<QK_ANY, true>), g, t, 0, s, D, n);
+        else hipLaunchKernelGGL((k_first<QK_ANY, false>), g, t, 0, s, D, n);
+        break;
     }
 }
 
@@ -1635,6 +1718,7 @@ FastBatch *fast_batch_create(const FastDomain *doms, int n, int order, int fct, 
             b->has_sr = b->has_sr || f.storage == ICAR_ADV_STORE_BF16_SR ||
                         f.storage == ICAR_ADV_STORE_F16_SR;
             b->q16 = b->q16 || f.storage != ICAR_ADV_STORE_F32;
+            b->f16 = b->f16 || f.storage == ICAR_ADV_STORE_F16 || f.storage == ICAR_ADV_STORE_F16_SR;
         }
         cb += cell_blocks(d);
         b->cells += (double)st->N;
