@@ -689,7 +689,9 @@ __global__ void __launch_bounds__(Tile<TJ>::CNT) __attribute__((amdgpu_waves_per
     // loads use a scalar base and one VGPR offset
     const unsigned sk = (unsigned)nx, sj = (unsigned)nx * (unsigned)nz;
     const size_t N = (size_t)nx * ny * nz;
-    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    // ly is the same for the whole wave: as a scalar, every row offset below
+    // is one too, which keeps the LDS addresses out of the vector registers
+    const int lx = threadIdx.x & 63, ly = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int gi = tx * CTI - CH + lx;
     const int j0 = ty * TJ - CH;  // global row of staged row 0
     SpecPass p = D.pass[pass - 1][s];
