@@ -491,6 +491,7 @@ __global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ 
             if (s0 + b < nspec) {
                 SpecPass p = sp[s0 + b];
                 if (QK == QK_F32) p.kind0 = ICAR_ADV_STORE_F32;  // an FP32-only batch: no 16-bit code
+                p.kind0 = __builtin_amdgcn_readfirstlane(p.kind0);
                 rc[b] = ld_w<QK>(p.q0, c, p.kind0);
                 if (!ring) {
                     rw[b] = ld_w<QK>(p.q0, c - 1, p.kind0);
@@ -508,8 +509,30 @@ __global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ 
             int kind = ICAR_ADV_STORE_F32;
             float sc = 1.0f;
             if (QK != QK_F32 && s0 + b < nspec) {
-                kind = sp[s0 + b].kind0;
+                kind = __builtin_amdgcn_readfirstlane(sp[s0 + b].kind0);
                 sc = sp[s0 + b].scale0;
+            }
+            if (QK == QK_BF16) {
+                // one branch per species (the kind is uniform): FP32 words as
+                // they are, BF16 ones shifted into place and scaled
+                if (kind == ICAR_ADV_STORE_F32) {
+                    qc[b] = __uint_as_float(rc[b]);
+                    qw[b] = __uint_as_float(rw[b]);
+                    qe[b] = __uint_as_float(re[b]);
+                    qs[b] = __uint_as_float(rs[b]);
+                    qn[b] = __uint_as_float(rn[b]);
+                    qu[b] = __uint_as_float(ru[b]);
+                    qd[b] = __uint_as_float(rd[b]);
+                } else {
+                    qc[b] = __uint_as_float(rc[b] << 16) * sc;
+                    qw[b] = __uint_as_float(rw[b] << 16) * sc;
+                    qe[b] = __uint_as_float(re[b] << 16) * sc;
+                    qs[b] = __uint_as_float(rs[b] << 16) * sc;
+                    qn[b] = __uint_as_float(rn[b] << 16) * sc;
+                    qu[b] = __uint_as_float(ru[b] << 16) * sc;
+                    qd[b] = __uint_as_float(rd[b] << 16) * sc;
+                }
+                continue;
             }
             qc[b] = cv_w<QK>(rc[b], c, kind, sc);
             qw[b] = cv_w<QK>(rw[b], c - 1, kind, sc);
@@ -561,6 +584,7 @@ __global__ void __launch_bounds__(FBX *FBY) k_first(const DomDesc *__restrict__ 
             if (s >= nspec) continue;
             SpecPass p = sp[s];
             if (QK == QK_F32) p.kindo = ICAR_ADV_STORE_F32;
+            p.kindo = __builtin_amdgcn_readfirstlane(p.kindo);
             const uint32_t rnd = is_sr(p.kindo) ? sr_bits((uint32_t)c, s, tstep) : 0u;
             st_w<QK>(p.out, c, p.kindo, p.inv_scaleo, qo[b], rnd);
         }
@@ -696,6 +720,8 @@ __global__ void __launch_bounds__(Tile<TJ>::CNT) __attribute__((amdgpu_waves_per
     const int j0 = ty * TJ - CH;  // global row of staged row 0
     SpecPass p = D.pass[pass - 1][s];
     if (QK == QK_F32) p.kind0 = p.kindo = ICAR_ADV_STORE_F32;  // an FP32-only batch: no 16-bit code
+    p.kind0 = __builtin_amdgcn_readfirstlane(p.kind0);  // uniform: branches on the kinds are scalar
+    p.kindo = __builtin_amdgcn_readfirstlane(p.kindo);
     const bool same = p.q1 == nullptr;
     const unsigned tstep = is_sr(p.kindo) ? *D.step : 0u;
     const float eps = 1e-10f, eps15 = 1e-15f;
