@@ -150,6 +150,23 @@ def test_mixed_storage_matches_emulated_oracle(oracle32, storage):
     assert case.domain.storage["qc"][0] == storage
 
 
+@pytest.mark.parametrize("storage", KINDS16)
+@pytest.mark.parametrize("fct", [True, False])
+def test_mixed_storage_order3_matches_emulated_oracle(oracle32, storage, fct):
+    """Order 3: the first corrective pass reads the 16-bit q0 and writes an
+    FP32 intermediate, the second reads both and writes the 16-bit result.
+    Each 16-bit class of the kernels (the BF16 kinds beside FP32, any kind
+    with FP16) against the FP32 oracle stepped on the 16-bit values, bit for
+    bit, with and without FCT (without it species can go negative)."""
+    case = synthetic.config4(storage=storage, nx=70, ny=36, nz=14, steps=6)
+    case.options.adv_options.mpdata_order = 3
+    case.options.adv_options.MPDATA_FCT = fct
+    ref = emulate(oracle32, case, case.steps)
+    got = run_gpu(case, case.steps)
+    for n in ref:
+        np.testing.assert_array_equal(got[n], ref[n], err_msg=n)
+
+
 def test_mixed_c4_error_against_fp64(oracle64, capsys):
     """The 256x64x20, 12-species domain with its hydrometeors in each 16-bit
     kind, 200 steps, against the FP64 oracle: per-field normalised max and
