@@ -131,6 +131,16 @@ __device__ __forceinline__ uint32_t sr_bits(uint32_t c, int s, uint32_t t)
     return x & 0xFFFFu;
 }
 
+// BF16 bits of x: stochastic rounding with the 16 random bits rnd (sr), else
+// round to nearest even; infinities and NaN are kept (a NaN stays a NaN)
+__device__ __forceinline__ uint16_t bf16_of(float x, bool sr, uint32_t rnd)
+{
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += sr ? rnd : 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
 // value -> 16-bit storage of value * inv_scale.  The plain kinds round to
 // nearest even; the _SR kinds round up with probability equal to the
 // discarded fraction of an ulp (rnd: 16 uniform bits), so that increments
@@ -139,15 +149,7 @@ __device__ __forceinline__ uint32_t sr_bits(uint32_t c, int s, uint32_t t)
 __device__ __forceinline__ uint16_t to16(float v, int kind, float inv_scale, uint32_t rnd)
 {
     const float x = v * inv_scale;
-    if (is_bf16(kind)) {
-        uint32_t u = __float_as_uint(x);
-        if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-        if (kind == ICAR_ADV_STORE_BF16_SR)
-            u += rnd;                         // then truncate: stochastic rounding
-        else
-            u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-        return (uint16_t)(u >> 16);
-    }
+    if (is_bf16(kind)) return bf16_of(x, kind == ICAR_ADV_STORE_BF16_SR, rnd);
     if (kind == ICAR_ADV_STORE_F16_SR) {
         const float ax = fabsf(x);
         if (ax >= 6.103515625e-05f) {  // FP16 normal range: 13 mantissa bits are dropped
@@ -216,16 +218,6 @@ __device__ __forceinline__ float cv_w_uniform(uint32_t w, size_t i, int kind, fl
 {
     if (QK == QK_ANY) return cvt_q_uniform(w, i, kind, scale);
     return cv_w<QK>(w, i, kind, scale);
-}
-
-// BF16 bits of x: stochastic rounding with the 16 random bits rnd (sr), else
-// round to nearest even; infinities and NaN are kept (to16)
-__device__ __forceinline__ uint16_t bf16_of(float x, bool sr, uint32_t rnd)
-{
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-    u += sr ? rnd : 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
 }
 
 template <int QK>
