@@ -167,6 +167,59 @@ def test_mixed_storage_order3_matches_emulated_oracle(oracle32, storage, fct):
         np.testing.assert_array_equal(got[n], ref[n], err_msg=n)
 
 
+@pytest.mark.parametrize("order", [2, 3])
+def test_every_kind_in_one_batch_matches_emulated_oracle(oracle32, order):
+    """One domain whose species use every storage kind at once (FP32, BF16,
+    BF16 stochastic, FP16, FP16 stochastic): the kernels for any kind, where
+    each block branches on its species' kind, against the FP32 oracle
+    stepped on the stored values, bit for bit."""
+    case = synthetic.config4(storage=_lib.STORE_BF16_SR, nx=80, ny=36, nz=16, steps=6)
+    case.options.adv_options.mpdata_order = order
+    d = case.domain
+    kinds = (_lib.STORE_BF16, _lib.STORE_F16, _lib.STORE_F16_SR, _lib.STORE_BF16_SR, None)
+    for idx, n in enumerate([n for n in d.species if n not in ("qv", "theta")]):
+        kind = kinds[idx % len(kinds)]
+        if kind is None:
+            d.storage.pop(n, None)  # FP32 between 16-bit species
+        else:
+            d.storage[n] = (kind, synthetic.pow2_scale(d.species[n], kind))
+    assert {k for k, _ in d.storage.values()} == set(KINDS16)
+    ref = emulate(oracle32, case, case.steps)
+    got = run_gpu(case, case.steps)
+    for n in ref:
+        np.testing.assert_array_equal(got[n], ref[n], err_msg=n)
+
+
+def test_ensemble_of_storage_classes_matches_each_member(oracle32):
+    """An ensemble whose members hold an FP32-only, a BF16 and an FP16
+    species set runs the kernels of the widest class for all of them; each
+    member equals its own run (and so, through the tests above, the
+    oracle), bit for bit."""
+    from hicar_amd.advection import Ensemble
+    cases = [synthetic.config4(storage=st, nx=nx, ny=30, nz=12, steps=4)
+             for st, nx in ((_lib.STORE_BF16_SR, 70), (_lib.STORE_F16, 54), (_lib.STORE_BF16, 40))]
+    cases[2].domain.storage.clear()  # every species FP32
+    want = []
+    for c in cases:
+        init = {n: a.copy() for n, a in c.domain.species.items()}
+        want.append(run_gpu(c, c.steps))
+        for n, a in init.items():
+            c.domain.species[n][...] = a
+    members = [MpdataAdvection(c.domain, c.options) for c in cases]
+    ens = Ensemble(members)
+    try:
+        ens.run([c.dt for c in cases], cases[0].steps, 1, 2, graph=True)
+        ens.sync()
+        for m, c, w in zip(members, cases, want):
+            m.download()
+            for n in w:
+                np.testing.assert_array_equal(c.domain.species[n], w[n], err_msg=n)
+    finally:
+        ens.close()
+        for m in members:
+            m.close()
+
+
 def test_mixed_c4_error_against_fp64(oracle64, capsys):
     """The 256x64x20, 12-species domain with its hydrometeors in each 16-bit
     kind, 200 steps, against the FP64 oracle: per-field normalised max and
