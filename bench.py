@@ -296,9 +296,9 @@ def read_valu(kernel: str, workload: str = "c2_1024x1024x30"):
 
 
 def host_cores() -> dict:
-    """Threads this process may use (the CPU share of the host, at most 16
-    on a one-GPU box) and the host's own counts (logical CPUs, physical cores
-    as lscpu lists them)."""
+    """Threads this process may use (its CPU affinity, the host's share for
+    one GPU) and the host's own counts (logical CPUs, physical cores as lscpu
+    lists them)."""
     import subprocess
     try:
         share = len(os.sched_getaffinity(0))
@@ -513,8 +513,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(every GPU box of this build has one GPU; the multi-GPU decomposition is a "
-              "design, not part of this build)", file=sys.stderr)
+              "(the library runs one domain or ensemble per GPU; the decomposition over "
+              "several GPUs is designed but not implemented)", file=sys.stderr)
         return 2
     if args.workload == "c2":
         out = c2_line(args, cpu=not args.no_cpu)

@@ -13,7 +13,7 @@ level moves by kelvins per step while its limiter factors are ratios of
 differences of ~370 K values (FCT_core:103-115, the third pass taking its
 bounds from q0 = q1, adv_mpdata.f90:388-397).  Last-bit differences there
 feed back step after step.  It is a property of the reference's FP32
-arithmetic, not of this build: the GPU path equals the FP32 oracle bit for
+arithmetic, not of the GPU kernels: the GPU path equals the FP32 oracle bit for
 bit.  An FP32 build with FMA contraction moves the same cells the same way
 and nothing else, which is why the kernels keep the reference's operation
 order instead of tolerating reordering.
