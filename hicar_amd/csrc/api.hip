@@ -1996,17 +1996,19 @@ void ens_free(Ensemble *e)
 
 int ens_steps(Ensemble *e, const float *dt, int nsteps, int bc_x, int bc_y)
 {
+    if (bc_x < 0 || bc_x > 2 || (bc_y != 0 && bc_y != 2))
+        return ens_err(e, ICAR_ADV_EINVAL, "bc_x must be 0/1/2 and bc_y 0/2");
     for (int st = 0; st < nsteps; st++) {
         std::string m;
         int rc = fast_batch_advect(e->batch, e->stream, dt, timed_fn(e), &m);
         if (rc) return ens_err(e, rc, "%s", m.c_str());
         for (size_t k = 0; k < e->hs.size(); k++) {
-            Handle *h = e->hs[k];
-            h->last_dt = dt[k];
-            h->have_dt = true;
-            rc = apply_bc_on(h, e, e->stream, bc_x, bc_y);
-            if (rc) return ens_err(e, rc, "%s", h->err.c_str());
+            e->hs[k]->last_dt = dt[k];
+            e->hs[k]->have_dt = true;
         }
+        // the ring conditions of every member's species: two launches
+        timed(e, K_BC, 0.0, [&] { rc = fast_batch_bc(e->batch, e->stream, bc_x, bc_y); });
+        if (rc) return ens_err(e, rc, "ring conditions: %s", hipGetErrorString((hipError_t)rc));
     }
     hipError_t x = hipGetLastError();
     if (x != hipSuccess) return ens_err(e, (int)x, "launch failed: %s", hipGetErrorString(x));

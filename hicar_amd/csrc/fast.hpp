@@ -62,6 +62,9 @@ void fast_batch_config(const FastBatch *b, int *corr_cfg, int *ksplit);
 using TimedFn = std::function<void(int id, double bytes, const std::function<void()> &)>;
 int fast_batch_advect(FastBatch *b, hipStream_t s, const float *dt, const TimedFn &timed,
                       std::string *err);
+// The ring conditions (bc_x 0/1/2, bc_y 0/2, as icar_adv_apply_bc) on the
+// current buffer of every species of every domain of the batch: two launches.
+int fast_batch_bc(const FastBatch *b, hipStream_t s, int bc_x, int bc_y);
 
 // FP32 <-> scaled BF16/FP16 conversion and ring conditions for 16-bit storage.
 void launch_pack(hipStream_t s, size_t n, const float *src, void *dst, int storage, int pow2);

@@ -1191,6 +1191,59 @@ __global__ void __launch_bounds__(Tile<TJ>::CNT) __attribute__((amdgpu_waves_per
     }
 }
 
+// The same ring conditions for every species of every domain of a batch: one
+// launch for the x sides and one for the y sides (the y sides copy the
+// corners the x sides have written), each over a table of (domain, species)
+// entries with the species' current buffer; a block row per entry.  The
+// conditions copy the stored bits, so one kernel serves every storage kind.
+struct BcEntry {
+    Dims d;
+    void *q;
+    int esz;  // bytes per element: 4 (FP32) or 2 (16-bit kinds)
+};
+
+template <typename E>
+__device__ __forceinline__ void bc_x_cells(E *q, const Dims &d, int k, int j, int bc_x)
+{
+    if (bc_x == 1) {
+        q[ix(d, d.nx - 1, k, j)] = q[ix(d, d.nx - 2, k, j)];
+    } else {
+        q[ix(d, 0, k, j)] = q[ix(d, d.nx - 2, k, j)];
+        q[ix(d, d.nx - 1, k, j)] = q[ix(d, 1, k, j)];
+    }
+}
+
+template <typename E>
+__device__ __forceinline__ void bc_y_cells(E *q, const Dims &d, int i, int k)
+{
+    q[ix(d, i, k, 0)] = q[ix(d, i, k, d.ny - 2)];
+    q[ix(d, i, k, d.ny - 1)] = q[ix(d, i, k, 1)];
+}
+
+__global__ void __launch_bounds__(256) k_bc_x_all(const BcEntry *__restrict__ tab, int bc_x)
+{
+    const BcEntry e = tab[blockIdx.y];
+    const unsigned t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= (unsigned)e.d.ny * (unsigned)e.d.nz) return;
+    const int k = (int)(t % (unsigned)e.d.nz), j = (int)(t / (unsigned)e.d.nz);
+    if (e.esz == 4)
+        bc_x_cells(static_cast<uint32_t *>(e.q), e.d, k, j, bc_x);
+    else
+        bc_x_cells(static_cast<uint16_t *>(e.q), e.d, k, j, bc_x);
+}
+
+__global__ void __launch_bounds__(256) k_bc_y_all(const BcEntry *__restrict__ tab)
+{
+    const BcEntry e = tab[blockIdx.y];
+    const unsigned t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= (unsigned)e.d.nx * (unsigned)e.d.nz) return;
+    const int i = (int)(t % (unsigned)e.d.nx), k = (int)(t / (unsigned)e.d.nx);
+    if (e.esz == 4)
+        bc_y_cells(static_cast<uint32_t *>(e.q), e.d, i, k);
+    else
+        bc_y_cells(static_cast<uint16_t *>(e.q), e.d, i, k);
+}
+
 unsigned grid_for(size_t n)
 {
     size_t b = (n + 255) / 256;
@@ -1235,6 +1288,9 @@ struct FastBatch {
     bool tuned = false;  // launch configuration chosen (autotuned or fixed by the environment)
     bool dirty = true;   // device tables out of date
     bool has_sr = false; // a species is stored with stochastic rounding
+    BcEntry *d_bc = nullptr;  // ring-condition table: [2 parities][n_bc] (fast_batch_bc)
+    int n_bc = 0;
+    unsigned bc_gx = 0, bc_gy = 0;  // blocks per entry for the x and y sides
     bool q16 = false;    // a species is stored in 16 bits
     bool f16 = false;    // a species is stored in FP16 (kernels for any kind)
 };
@@ -1754,16 +1810,49 @@ FastBatch *fast_batch_create(const FastDomain *doms, int n, int order, int fct, 
     set_ksplit(b, b->nks);
     if (hipMalloc(&b->d_desc, 2 * (size_t)n * sizeof(DomDesc)) != hipSuccess) {
         *err = "hipMalloc of the domain table failed";
-        delete b;
+        fast_batch_destroy(b);
         return nullptr;
     }
+    // ring-condition table: every species of every domain, at both parities
+    std::vector<BcEntry> bc[2];
+    for (int k = 0; k < n; k++) {
+        FastState *st = doms[k].st;
+        const Dims &d = st->d;
+        b->bc_gx = std::max(b->bc_gx, ((unsigned)d.ny * (unsigned)d.nz + 255u) / 256u);
+        b->bc_gy = std::max(b->bc_gy, ((unsigned)d.nx * (unsigned)d.nz + 255u) / 256u);
+        for (int s = 0; s < (int)st->sp.size(); s++)
+            for (int par = 0; par < 2; par++)
+                bc[par].push_back({d, home(st, s, par), (int)kind_bytes(st->sp[s].storage)});
+    }
+    b->n_bc = (int)bc[0].size();
+    if (b->n_bc > 0) {
+        const size_t nb = (size_t)b->n_bc * sizeof(BcEntry);
+        if (hipMalloc(&b->d_bc, 2 * nb) != hipSuccess ||
+            hipMemcpy(b->d_bc, bc[0].data(), nb, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(b->d_bc + b->n_bc, bc[1].data(), nb, hipMemcpyHostToDevice) != hipSuccess) {
+            *err = "the ring-condition table could not be set up on the device";
+            fast_batch_destroy(b);
+            return nullptr;
+        }
+    }
     return b;
+}
+
+int fast_batch_bc(const FastBatch *b, hipStream_t s, int bc_x, int bc_y)
+{
+    if (b->n_bc == 0 || (bc_x == 0 && bc_y != 2)) return 0;
+    const BcEntry *tab = b->d_bc + (size_t)b->doms[0].st->cur * b->n_bc;
+    if (bc_x == 1 || bc_x == 2)
+        hipLaunchKernelGGL(k_bc_x_all, dim3(b->bc_gx, b->n_bc), dim3(256), 0, s, tab, bc_x);
+    if (bc_y == 2) hipLaunchKernelGGL(k_bc_y_all, dim3(b->bc_gy, b->n_bc), dim3(256), 0, s, tab);
+    return (int)hipGetLastError();
 }
 
 void fast_batch_destroy(FastBatch *b)
 {
     if (!b) return;
     (void)hipFree(b->d_desc);
+    (void)hipFree(b->d_bc);
     delete b;
 }
 
@@ -1869,6 +1958,7 @@ __global__ void k_bc16_y(Dims d, uint16_t *q)
     q[ix(d, i, k, 0)] = q[ix(d, i, k, d.ny - 2)];
     q[ix(d, i, k, d.ny - 1)] = q[ix(d, i, k, 1)];
 }
+
 
 }  // namespace
 
