@@ -1825,20 +1825,27 @@ int icar_adv_run(void *handle, float dt, int nsteps, int bc_x, int bc_y)
     int rc = check_handle(handle, &h);
     if (rc) return rc;
     if (nsteps < 0) return set_err(h, ICAR_ADV_EINVAL, "nsteps < 0");
+    const bool fused = h->fast && h->o.kernel_variant == 1 && !h->o.rk3 && h->nspecies > 0 &&
+                       h->o.mpdata_order >= 1;
+    if (bc_x < 0 || bc_x > 2 || (bc_y != 0 && bc_y != 2))
+        return set_err(h, ICAR_ADV_EINVAL, "bc_x must be 0/1/2 and bc_y 0/2");
     auto steps = [&]() -> int {
         for (int s = 0; s < nsteps; s++) {
             int r = do_advect(h, dt);
             if (r) return r;
-            r = icar_adv_apply_bc(handle, bc_x, bc_y);
-            if (r) return r;
+            if (fused && h->batch) {  // every species' ring conditions in two launches
+                timed(h, K_BC, 0.0, [&] { r = fast_batch_bc(h->batch, h->stream, bc_x, bc_y); });
+                if (r) return set_err(h, r, "ring conditions: %s", hipGetErrorString((hipError_t)r));
+            } else {
+                r = icar_adv_apply_bc(handle, bc_x, bc_y);
+                if (r) return r;
+            }
         }
         return 0;
     };
     // graph capture needs every launch to be asynchronous: not with the
     // profiling events or the debug check's host copies
     if (!h->use_graph || h->prof || h->o.debug || nsteps == 0) return steps();
-    const bool fused = h->fast && h->o.kernel_variant == 1 && !h->o.rk3 && h->nspecies > 0 &&
-                       h->o.mpdata_order >= 1;
     const int p0 = parity(h);
     if (h->gexec && h->g_dt == dt && h->g_steps == nsteps && h->g_bcx == bc_x &&
         h->g_bcy == bc_y && h->g_par0 == p0) {
