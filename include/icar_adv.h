@@ -209,9 +209,11 @@ int icar_adv_divergence(void *handle, double out[2]);
  * share the array, so calls that overlap (on different streams) race.  The
  * first call on a device looks up the array's device address
  * (hipGetSymbolAddress): make it before capturing one.
+ * The reduction takes at most 1024 blocks for any grid (each block strides
+ * over its share of the cells), so every grid fits the array.
  * Errors (icar_adv_last_error(NULL) holds the message): ICAR_ADV_EINVAL for
- * nx, ny or nz below 3 or a NULL array, ICAR_ADV_ERANGE if the reduction
- * would need more partial sums than the array holds. */
+ * nx, ny or nz below 3 or a NULL array; ICAR_ADV_ERANGE, should the
+ * reduction ever take more blocks than the array holds. */
 #define ICAR_ADV_NOALLOC_PARTIALS 1024
 int icar_adv_mass_budget_noalloc(int nx, int ny, int nz, const float *q, const float *jacobian,
                                  const float *density, const float *advection_dz, double *out,

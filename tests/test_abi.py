@@ -104,3 +104,29 @@ def test_fortran_interface_binds_declared_symbols(tmp_path):
         pytest.skip("no Fortran compiler in this environment")
     subprocess.run([fc, "-c", src, "-module-dir", str(tmp_path), "-o", str(tmp_path / "m.o")],
                    check=True, timeout=120)
+
+
+def test_extension_entry_points_check_arguments_without_device():
+    """The division check, the allocation-free budget, the divergence result
+    and the compensation switch refuse bad arguments in host code, before any
+    device call."""
+    import numpy as np
+    lib = _lib.load()
+    bad, first = ctypes.c_ulonglong(), ctypes.c_ulonglong()
+    sig = 1 << 23
+    for d0, n in ((0, 0), (sig, 1), (sig - 4, 5)):
+        assert lib.icar_adv_verify_division(d0, n, ctypes.byref(bad), ctypes.byref(first)) \
+            == _lib.EINVAL, (d0, n)
+        assert b"verify_division" in lib.icar_adv_last_error(None)
+    assert lib.icar_adv_verify_division(0, 1, None, ctypes.byref(first)) == _lib.EINVAL
+
+    fp = ctypes.POINTER(ctypes.c_float)
+    a = np.zeros(8, np.float32).ctypes.data_as(fp)
+    out = (ctypes.c_double * 1)()
+    assert lib.icar_adv_mass_budget_noalloc(2, 8, 8, a, a, a, a, out, None) == _lib.EINVAL
+    assert lib.icar_adv_mass_budget_noalloc(8, 8, 8, a, None, a, a, out, None) == _lib.EINVAL
+    assert b"noalloc" in lib.icar_adv_last_error(None)
+
+    res = (ctypes.c_double * 2)()
+    assert lib.icar_adv_divergence(None, res) == _lib.EINVAL
+    assert lib.icar_adv_set_compensation(None, 1) == _lib.EINVAL
