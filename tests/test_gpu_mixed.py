@@ -437,3 +437,26 @@ def test_compensated_divergence_against_fp64(oracle64, capsys):
             print(f"\n[compensation] {kind:7s} {n:6s} max err plain {e[False]:.3e} "
                   f"compensated {e[True]:.3e}", end="")
         print()
+
+
+@pytest.mark.parametrize("cfg,ksplit", [(0, 1), (3, 3), (5, 1), (7, 2), (1, 4)])
+def test_mixed_storage_every_launch_configuration(oracle32, monkeypatch, cfg, ksplit):
+    """The BF16 class of the corrective kernel under each register budget,
+    prefetch choice and tile height (60x16, 60x32), with the columns split
+    into chunks (whose warm-up levels read the 16-bit q0), against the
+    emulated oracle bit for bit; order 3 so that both corrective passes run."""
+    monkeypatch.setenv("ICAR_ADV_CORR_CFG", str(cfg))
+    monkeypatch.setenv("ICAR_ADV_KSPLIT", str(ksplit))
+    case = synthetic.config4(storage=_lib.STORE_BF16_SR, nx=130, ny=70, nz=20, steps=4)
+    case.options.adv_options.mpdata_order = 3
+    ref = emulate(oracle32, case, case.steps)
+    st = MpdataAdvection(case.domain, case.options)
+    try:
+        st.run(case.dt, case.steps, case.bc_x, case.bc_y)
+        st.download()
+        assert tuple(st.launch_config()) == (cfg, ksplit)
+        got = {n: case.domain.species[n].copy() for n in st.names}
+    finally:
+        st.close()
+    for n in ref:
+        np.testing.assert_array_equal(got[n], ref[n], err_msg=n)
