@@ -460,3 +460,24 @@ def test_mixed_storage_every_launch_configuration(oracle32, monkeypatch, cfg, ks
         st.close()
     for n in ref:
         np.testing.assert_array_equal(got[n], ref[n], err_msg=n)
+
+
+@pytest.mark.parametrize("storage", [_lib.STORE_BF16_SR, _lib.STORE_F16_SR])
+def test_graph_replays_advance_the_stochastic_rounding(oracle32, storage):
+    """A recorded step loop with stochastically rounded species, replayed
+    three times with downloads between the replays: the step counter that
+    seeds the rounding advances inside the graph, so the result equals the
+    emulated oracle over all the steps, bit for bit."""
+    case = synthetic.config4(storage=storage, nx=70, ny=40, nz=12, steps=3)
+    ref = emulate(oracle32, case, 3 * case.steps)
+    st = MpdataAdvection(case.domain, case.options)
+    try:
+        st.set_graph(True)
+        for _ in range(3):
+            st.run(case.dt, case.steps, case.bc_x, case.bc_y)
+            st.download()  # host work between the replays
+        got = {n: case.domain.species[n].copy() for n in st.names}
+    finally:
+        st.close()
+    for n in ref:
+        np.testing.assert_array_equal(got[n], ref[n], err_msg=n)
